@@ -1,0 +1,217 @@
+"""Benchmark of the directed Hausdorff-distance hot path on one MI355X.
+
+Workload (default): 10M x 10M 2D FP32 Gaussian mixture (64 components,
+sigma 0.01), B = A + (0.002, 0.001) with 1% uniform outliers. One step is one
+directed h(A -> B) over device-resident inputs through the C ABI
+(hd_directed_device), early-break variant; the plain (pseudo-random) visit
+order and the learned order are both timed, `value` is the learned order's
+A-points/s when the learned path is available, else the plain one's.
+
+Prints ONE JSON line. The roofline object is for the dominant kernel (the
+early-break slot kernel), its duration measured with HIP events on the
+library's stream; the CPU baseline is the CPU oracle (a C restatement of the
+reference's CPU early break) on a bounded sample of the same workload.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# VALU lane-op ceiling: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz.
+VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
+LANE_OPS_PER_EVAL = {2: 6, 3: 9}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def stage_times(lib, ctx, reset: bool) -> dict:
+    buf = C.create_string_buffer(1 << 16)
+    rc = lib.hd_context_stage_times(ctx.handle, buf, 1 << 16, 1 if reset else 0)
+    if rc != 0:
+        raise RuntimeError(lib.hd_last_error().decode())
+    return json.loads(buf.value.decode())
+
+
+def time_steps(hd, ctx, variant, da, db, steps, warmup, learned, model=None):
+    import torch
+
+    lib = hd.load()
+    for _ in range(warmup):
+        hd.directed_device(ctx, variant, da, db, seed=0, learned_order=learned, model=model)
+    torch.cuda.synchronize()
+    lib.hd_context_set_profiling(ctx.handle, 1)
+    stage_times(lib, ctx, reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    results = []
+    for _ in range(steps):
+        results.append(hd.directed_device(ctx, variant, da, db, seed=0, learned_order=learned,
+                                          model=model))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stages = stage_times(lib, ctx, reset=True)
+    lib.hd_context_set_profiling(ctx.handle, 0)
+    return (t1 - t0) * 1e3 / steps, results, stages
+
+
+def cpu_baseline(a: np.ndarray, b: np.ndarray, target_s: float = 12.0) -> dict:
+    """CPU oracle early break on a random sample of A against all of B."""
+    from oracle import oracle
+
+    cores = min(os.cpu_count() or 1, 16)
+    rng = np.random.default_rng(0)
+    k = min(100_000, a.shape[0])
+    idx = rng.choice(a.shape[0], k, replace=False)
+    r = oracle.eb_directed(a[idx], b, n_threads=cores, seed=0)
+    t = r.compute_ms / 1e3
+    if t < target_s / 4 and k < a.shape[0]:
+        k2 = int(min(a.shape[0], k * target_s / max(t, 1e-3)))
+        idx = rng.choice(a.shape[0], k2, replace=False)
+        r = oracle.eb_directed(a[idx], b, n_threads=cores, seed=0)
+        k = k2
+        t = r.compute_ms / 1e3
+    return {
+        "value": k / t,
+        "unit": "A-points/s",
+        "cores": int(r.n_threads),
+        "kind": "port",
+        "sample": f"{k} random A points (seed 0) vs all {b.shape[0]} B points, "
+                  f"directed A->B, {r.compared_pairs} d2 evaluations in {t:.2f} s",
+    }
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+
+    import hausdorffdistance_amd as hd
+
+    log(f"generating gmm {args.n} x {args.n}")
+    a, b = hd.gen.gmm_10m(args.n)
+    da = torch.from_numpy(a).cuda()
+    db = torch.from_numpy(b).cuda()
+    torch.cuda.synchronize()
+    ctx = hd.Context(0)
+    info = hd.device_info(0)
+    log(f"device {info.get('name')} {info.get('gcnArchName')} CUs={info.get('multiProcessorCount')}")
+
+    log("plain order")
+    ms_plain, res_plain, st_plain = time_steps(hd, ctx, hd.Variant.EB, da, db, args.steps,
+                                               args.warmup, learned=False)
+    r0 = res_plain[-1]
+    log(f"plain: {ms_plain:.3f} ms/step, cmax2 bits {r0.cmax2_bits} argmax {r0.argmax} "
+        f"evals {r0.evals}")
+    ms_learned = None
+    st_learned = None
+    r1 = None
+    if not args.no_learned:
+        try:
+            ms_learned, res_learned, st_learned = time_steps(
+                hd, ctx, hd.Variant.EB, da, db, args.steps, args.warmup, learned=True)
+            r1 = res_learned[-1]
+            log(f"learned: {ms_learned:.3f} ms/step, evals {r1.evals}")
+            if (r1.cmax2_bits, r1.argmax) != (r0.cmax2_bits, r0.argmax):
+                raise RuntimeError("learned order changed the result")
+        except hd.HDError as e:
+            log(f"learned order unavailable: {e}")
+            ms_learned = None
+
+    use_learned = ms_learned is not None
+    ms = ms_learned if use_learned else ms_plain
+    stages = st_learned if use_learned else st_plain
+    n_a = a.shape[0]
+    value = n_a / (ms / 1e3)
+
+    # Roofline of the dominant kernel (slot kernel of the plain order).
+    slot = st_plain.get("eb_slot", {})
+    calls = max(1, slot.get("calls", 1))
+    t_kernel = slot.get("ms", float("nan")) / calls / 1e3
+    evals = slot.get("evals", 0) / calls
+    points = slot.get("points", 0) / calls
+    bytes_alg = 8.0 * (points + evals)
+    achieved = bytes_alg / t_kernel / 1e9 if t_kernel > 0 else None
+    roofline = {
+        "bound": "hbm",
+        "kernel": "EbSlotKernel<2,8>",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS if achieved else None,
+        "traffic": None,
+        "algorithmic_bytes": bytes_alg,
+        "kernel_ms": t_kernel * 1e3,
+        "valu_frac": (evals * LANE_OPS_PER_EVAL[2] / t_kernel / VALU_LANE_OPS)
+        if t_kernel > 0 else None,
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(a, b)
+        log(f"cpu: {cpu['value']:.4g} A-points/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "A-points/sec, 10M x 10M 2D HD (directed, early break), learned-order vs plain",
+        "value": value,
+        "unit": "A-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"gmm_{args.n // 1_000_000}m_2d: A {n_a} points from a 64-component "
+                        f"Gaussian mixture (sigma 0.01), B = A + (0.002, 0.001) with 1% "
+                        f"uniform outliers; directed h(A->B), early-break variant",
+            "order": "learned" if use_learned else "plain",
+            "plain_ms_per_step": ms_plain,
+            "plain_value": n_a / (ms_plain / 1e3),
+            "learned_ms_per_step": ms_learned,
+            "learned_speedup": (ms_plain / ms_learned) if use_learned else None,
+            "cmax2_bits": r0.cmax2_bits,
+            "hd": r0.hd,
+            "argmax": r0.argmax,
+            "nn": r0.nn,
+            "evals_plain": r0.evals,
+            "evals_learned": r1.evals if r1 is not None else None,
+            "stages_ms": {k: v["ms"] / max(1, v["calls"]) for k, v in stages.items()},
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

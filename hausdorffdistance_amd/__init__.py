@@ -1,0 +1,26 @@
+"""MI355X-native exact Hausdorff distance (directed and symmetric).
+
+The compute lives in libhd.so (hand-written HIP for gfx950, C ABI in
+include/hd.h); this package is its Python host mirror of the reference's run
+interface (RunConfig / RunHausdorffDistance / HausdorffDistance).
+"""
+from ._lib import HDError, LIB_PATH, load  # noqa: F401
+from .api import (  # noqa: F401
+    Context,
+    DirectedResult,
+    Execution,
+    HausdorffDistance,
+    HausdorffDistanceEarlyBreak,
+    HausdorffDistanceGrid,
+    HausdorffDistanceHybrid,
+    HausdorffDistanceLbvh,
+    Result,
+    RunConfig,
+    Variant,
+    device_count,
+    device_info,
+    directed_device,
+    nn_device,
+    run_hausdorff_distance,
+)
+from . import gen  # noqa: F401

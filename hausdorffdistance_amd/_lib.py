@@ -1,0 +1,193 @@
+"""ctypes binding of libhd.so (include/hd.h, include/hd_gen.h).
+
+The library is the product: every HD computation goes through it. Loading
+fails loudly when it has not been built; there is no Python fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("HD_LIB", _PKG / "lib" / "libhd.so"))
+
+HD_OK = 0
+HD_EINVAL = -1
+HD_ENOMEM = -2
+HD_EDEVICE = -3
+HD_EUNSUPPORTED = -4
+HD_ETRUNC = -5
+
+HD_VARIANT_EB = 0
+HD_VARIANT_GRID = 1
+HD_VARIANT_LBVH = 2
+HD_VARIANT_HYBRID = 3
+
+HD_EXEC_CPU = 0
+HD_EXEC_GPU = 1
+
+HD_FLAG_SYMMETRIC = 0x1
+HD_FLAG_CHECK = 0x2
+HD_FLAG_LEARNED_ORDER = 0x4
+HD_FLAG_COUNT_EVALS = 0x8
+HD_FLAG_INPUT_ON_DEVICE = 0x10
+HD_FLAG_NO_SHUFFLE = 0x20
+
+HD_ABI_VERSION = 1
+
+
+class RunConfigC(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32),
+        ("variant", C.c_int32),
+        ("execution", C.c_int32),
+        ("n_dims", C.c_int32),
+        ("repeat", C.c_int32),
+        ("seed", C.c_int32),
+        ("parallelism", C.c_int32),
+        ("flags", C.c_uint32),
+        ("sample_rate", C.c_float),
+        ("n_points_cell", C.c_uint32),
+        ("max_hit", C.c_uint32),
+        ("eb_only_threshold", C.c_uint32),
+        ("auto_tune", C.c_int32),
+        ("reserved", C.c_int32),
+        ("model", C.c_void_p),
+    ]
+
+
+class DirectedResultC(C.Structure):
+    _fields_ = [
+        ("cmax2_bits", C.c_uint32),
+        ("argmax", C.c_uint32),
+        ("nn", C.c_uint32),
+        ("variant_used", C.c_int32),
+        ("hd", C.c_float),
+        ("reserved", C.c_float),
+        ("evals", C.c_uint64),
+        ("time_ms", C.c_double),
+    ]
+
+
+class ResultC(C.Structure):
+    _fields_ = [
+        ("hd", C.c_float),
+        ("cmax2_bits", C.c_uint32),
+        ("argmax", C.c_uint32),
+        ("nn", C.c_uint32),
+        ("direction", C.c_int32),
+        ("check_pass", C.c_int32),
+        ("n_repeats", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("ab", DirectedResultC),
+        ("ba", DirectedResultC),
+        ("avg_time_ms", C.c_double),
+        ("median_time_ms", C.c_double),
+        ("min_time_ms", C.c_double),
+    ]
+
+
+class MlpTrainConfigC(C.Structure):
+    _fields_ = [
+        ("steps", C.c_uint32),
+        ("batch", C.c_uint32),
+        ("n_samples", C.c_uint32),
+        ("seed", C.c_uint32),
+        ("lr", C.c_float),
+        ("label_index", C.c_int32),
+    ]
+
+
+# Every symbol include/hd.h and include/hd_gen.h declare, with its signature.
+_P = C.c_void_p
+_F = C.POINTER(C.c_float)
+_U32 = C.POINTER(C.c_uint32)
+_U64 = C.POINTER(C.c_uint64)
+SIGNATURES = {
+    # hd.h
+    "hd_last_error": (C.c_char_p, []),
+    "hd_run_config_init": (None, [C.POINTER(RunConfigC)]),
+    "hd_context_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "hd_context_destroy": (None, [_P]),
+    "hd_context_stream": (C.c_void_p, [_P]),
+    "hd_run_hausdorff_distance": (
+        C.c_int,
+        [_P, C.POINTER(RunConfigC), _P, C.c_uint64, _P, C.c_uint64, C.POINTER(ResultC),
+         C.c_char_p, C.c_uint64],
+    ),
+    "hd_directed_device": (
+        C.c_int,
+        [_P, C.c_int, C.c_int, _P, C.c_uint64, _P, C.c_uint64, C.c_uint32, C.c_int32, _P,
+         C.POINTER(DirectedResultC)],
+    ),
+    "hd_nn_device": (
+        C.c_int, [_P, C.c_int, C.c_int, _P, C.c_uint64, _P, C.c_uint64, _P, _P]),
+    "hd_batch_directed_device": (
+        C.c_int,
+        [_P, C.c_int, _P, _P, _P, _P, C.c_uint32, C.c_uint32, C.POINTER(DirectedResultC)],
+    ),
+    "hd_mlp_train_config_init": (None, [C.POINTER(MlpTrainConfigC)]),
+    "hd_mlp_train_device": (
+        C.c_int,
+        [_P, C.c_int, _P, C.c_uint64, _P, C.c_uint64, C.POINTER(MlpTrainConfigC),
+         C.POINTER(C.c_void_p), _F],
+    ),
+    "hd_mlp_predict_device": (C.c_int, [_P, _P, _P, C.c_uint64, _P]),
+    "hd_mlp_destroy": (None, [_P]),
+    "hd_mlp_n_params": (C.c_int, [C.c_int]),
+    "hd_mlp_get_params": (C.c_int, [_P, _F, _F, _F]),
+    "hd_mlp_loss_grad_device": (
+        C.c_int, [_P, C.c_int, _F, _F, _F, C.c_uint32, _F, _F]),
+    "hd_context_set_profiling": (C.c_int, [_P, C.c_int]),
+    "hd_context_stage_times": (C.c_int, [_P, C.c_char_p, C.c_uint64, C.c_int]),
+    "hd_device_count": (C.c_int, []),
+    "hd_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_uint64]),
+    # hd_gen.h
+    "hd_splitmix64_at": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+    "hd_gen_uniform": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int, _F]),
+    "hd_gen_gmm_pair": (
+        C.c_int,
+        [C.c_uint64, C.c_uint32, C.c_double, C.c_float, C.c_float, C.c_uint64,
+         C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, _F, _F],
+    ),
+    "hd_gen_sphere": (C.c_int, [C.c_uint64, C.c_uint64, C.c_double, C.c_double, _F]),
+    "hd_gen_batch_sizes": (C.c_int, [C.c_uint64, C.c_uint32, _U32, _U32]),
+    "hd_gen_batch_points": (
+        C.c_int, [C.c_uint64, C.c_uint32, C.c_double, _U32, _U32, _U64, _U64, _F, _F]),
+    "hd_gen_subsample": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, _U32]),
+}
+
+_lib = None
+
+
+class HDError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def load() -> C.CDLL:
+    """Loads libhd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m hausdorffdistance_amd.build` "
+            "(the HIP library is required; there is no fallback)"
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code: int) -> int:
+    if code != HD_OK and code != HD_ETRUNC:
+        msg = load().hd_last_error()
+        raise HDError(code, msg.decode() if msg else "")
+    return code

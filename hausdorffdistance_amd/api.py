@@ -1,0 +1,320 @@
+"""Host-side mirror of the reference's run interface, over libhd.so.
+
+Reference interface (pwrliang/HausdorffDistance):
+  * RunConfig            src/run_config.h:20-61 (variant, execution, n_dims,
+                         repeat, seed, parallelism, check, hybrid knobs)
+  * RunHausdorffDistance src/run_hausdorff_distance.cuh:5, .cu:53-402
+  * HausdorffDistance<T,N>::CalculateDistance / get_stats
+                         src/hausdorff_distance.h:15-42
+Same names (snake_case), same argument meaning; errors raise HDError with the
+library's code instead of aborting the process.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+from dataclasses import dataclass, field
+from enum import IntEnum
+from typing import Any, Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import HDError
+
+
+class Variant(IntEnum):
+    EB = _lib.HD_VARIANT_EB
+    GRID = _lib.HD_VARIANT_GRID
+    LBVH = _lib.HD_VARIANT_LBVH
+    HYBRID = _lib.HD_VARIANT_HYBRID
+
+    @staticmethod
+    def from_string(s: str) -> "Variant":
+        """CLI strings of reference src/main.cpp:76-94; "rt" is the BVH query
+        (a software LBVH here)."""
+        key = s.lower()
+        table = {"eb": Variant.EB, "rt": Variant.LBVH, "lbvh": Variant.LBVH,
+                 "hybrid": Variant.HYBRID, "grid": Variant.GRID}
+        if key not in table:
+            raise HDError(_lib.HD_EINVAL, f"Unknown variant: {s}")
+        return table[key]
+
+
+class Execution(IntEnum):
+    CPU = _lib.HD_EXEC_CPU
+    GPU = _lib.HD_EXEC_GPU
+
+    @staticmethod
+    def from_string(s: str) -> "Execution":
+        key = s.lower()
+        if key == "cpu":
+            return Execution.CPU
+        if key == "gpu":
+            return Execution.GPU
+        raise HDError(_lib.HD_EINVAL, f"Unknown execution: {s}")
+
+
+@dataclass
+class RunConfig:
+    """Reference RunConfig restricted to the hot path; defaults of
+    src/flags.cc:3-57 (check defaults to False here because the reference's
+    check re-runs its CPU path, which this library does not serve)."""
+
+    variant: Variant = Variant.EB
+    execution: Execution = Execution.GPU
+    n_dims: int = 0
+    repeat: int = 5
+    seed: int = 0
+    parallelism: int = -1
+    check: bool = False
+    symmetric: bool = False
+    learned_order: bool = False
+    shuffle: bool = True
+    sample_rate: float = 0.01
+    n_points_cell: int = 8
+    max_hit: int = 256
+    eb_only_threshold: int = 0
+    auto_tune: bool = False
+
+    def to_c(self) -> _lib.RunConfigC:
+        c = _lib.RunConfigC()
+        _lib.load().hd_run_config_init(C.byref(c))
+        c.variant = int(self.variant)
+        c.execution = int(self.execution)
+        c.n_dims = int(self.n_dims)
+        c.repeat = int(self.repeat)
+        c.seed = int(self.seed)
+        c.parallelism = int(self.parallelism)
+        flags = 0
+        if self.check:
+            flags |= _lib.HD_FLAG_CHECK
+        if self.symmetric:
+            flags |= _lib.HD_FLAG_SYMMETRIC
+        if self.learned_order:
+            flags |= _lib.HD_FLAG_LEARNED_ORDER
+        if not self.shuffle:
+            flags |= _lib.HD_FLAG_NO_SHUFFLE
+        c.flags = flags
+        c.sample_rate = float(self.sample_rate)
+        c.n_points_cell = int(self.n_points_cell)
+        c.max_hit = int(self.max_hit)
+        c.eb_only_threshold = int(self.eb_only_threshold)
+        c.auto_tune = 1 if self.auto_tune else 0
+        return c
+
+
+@dataclass
+class DirectedResult:
+    cmax2_bits: int
+    argmax: int
+    nn: int
+    variant_used: int
+    hd: float
+    evals: int
+    time_ms: float
+
+    @property
+    def cmax2(self) -> float:
+        return float(np.array([self.cmax2_bits], dtype=np.uint32).view(np.float32)[0])
+
+    @staticmethod
+    def from_c(r: _lib.DirectedResultC) -> "DirectedResult":
+        return DirectedResult(r.cmax2_bits, r.argmax, r.nn, r.variant_used, r.hd, r.evals,
+                              r.time_ms)
+
+
+@dataclass
+class Result:
+    hd: float
+    cmax2_bits: int
+    argmax: int
+    nn: int
+    direction: int
+    check_pass: int
+    ab: DirectedResult
+    ba: DirectedResult
+    avg_time_ms: float
+    median_time_ms: float
+    min_time_ms: float
+    stats: dict = field(default_factory=dict)
+
+
+class Context:
+    """One HIP stream plus cached device buffers on one device."""
+
+    def __init__(self, device: int = 0):
+        lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(lib.hd_context_create(device, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self) -> C.c_void_p:
+        if self._h is None:
+            raise HDError(_lib.HD_EINVAL, "context is closed")
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return int(_lib.load().hd_context_stream(self.handle) or 0)
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.load().hd_context_destroy(self._h)
+            self._h = None
+
+    def __enter__(self) -> "Context":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _is_torch_cuda(x: Any) -> bool:
+    return type(x).__module__.startswith("torch") and getattr(x, "is_cuda", False)
+
+
+def _points(x: Any, n_dims: int):
+    """Returns (pointer, n, on_device, keepalive) for an (n, d) float32 set."""
+    if _is_torch_cuda(x):
+        if x.dtype.__repr__() != "torch.float32" or x.dim() != 2 or x.shape[1] != n_dims:
+            raise HDError(_lib.HD_EINVAL, f"expected a float32 (n, {n_dims}) CUDA tensor")
+        if not x.is_contiguous():
+            x = x.contiguous()
+        return C.c_void_p(x.data_ptr()), int(x.shape[0]), True, x
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    if a.ndim != 2 or a.shape[1] != n_dims:
+        raise HDError(_lib.HD_EINVAL, f"expected an (n, {n_dims}) array, got {a.shape}")
+    return C.c_void_p(a.ctypes.data), int(a.shape[0]), False, a
+
+
+def run_hausdorff_distance(config: RunConfig, points_a, points_b,
+                           ctx: Optional[Context] = None,
+                           stats_cap: int = 1 << 20) -> Result:
+    """RunHausdorffDistance (reference src/run_hausdorff_distance.cu:53-402)
+    for in-memory inputs: `repeat` timed runs, optional check, stats JSON."""
+    lib = _lib.load()
+    if config.n_dims not in (2, 3):
+        raise HDError(_lib.HD_EINVAL,
+                      "Wrong number of dimensions, which can only be 2 or 3")
+    pa, na, dev_a, ka = _points(points_a, config.n_dims)
+    pb, nb, dev_b, kb = _points(points_b, config.n_dims)
+    if dev_a != dev_b:
+        raise HDError(_lib.HD_EINVAL, "both sets must be on the host or both on the device")
+    c = config.to_c()
+    if dev_a:
+        c.flags |= _lib.HD_FLAG_INPUT_ON_DEVICE
+    out = _lib.ResultC()
+    buf = C.create_string_buffer(stats_cap)
+    own = ctx is None
+    ctx = ctx or Context(0)
+    try:
+        rc = lib.hd_run_hausdorff_distance(ctx.handle, C.byref(c), pa, na, pb, nb,
+                                           C.byref(out), buf, stats_cap)
+        _lib.check(rc)
+    finally:
+        if own:
+            ctx.close()
+    stats = json.loads(buf.value.decode()) if buf.value else {}
+    return Result(out.hd, out.cmax2_bits, out.argmax, out.nn, out.direction, out.check_pass,
+                  DirectedResult.from_c(out.ab), DirectedResult.from_c(out.ba),
+                  out.avg_time_ms, out.median_time_ms, out.min_time_ms, stats)
+
+
+def directed_device(ctx: Context, variant: Variant, points_a, points_b, *,
+                    n_dims: Optional[int] = None, seed: int = 0, shuffle: bool = True,
+                    learned_order: bool = False, model=None) -> DirectedResult:
+    """h(A -> B) on device-resident (torch CUDA) inputs."""
+    nd = n_dims or int(points_a.shape[1])
+    pa, na, da, _ = _points(points_a, nd)
+    pb, nb, db, _ = _points(points_b, nd)
+    if not (da and db):
+        raise HDError(_lib.HD_EINVAL, "directed_device needs CUDA tensors")
+    flags = 0
+    if not shuffle:
+        flags |= _lib.HD_FLAG_NO_SHUFFLE
+    if learned_order:
+        flags |= _lib.HD_FLAG_LEARNED_ORDER
+    out = _lib.DirectedResultC()
+    mh = getattr(model, "handle", None)
+    _lib.check(_lib.load().hd_directed_device(ctx.handle, int(variant), nd, pa, na, pb, nb,
+                                              flags, seed, mh, C.byref(out)))
+    return DirectedResult.from_c(out)
+
+
+def nn_device(ctx: Context, index: Variant, queries, refs, out_d2, out_idx) -> None:
+    """Exact NN of every query (CUDA tensors; out_d2 float32, out_idx int32)."""
+    nd = int(queries.shape[1])
+    pq, nq, _, _ = _points(queries, nd)
+    pr, nr, _, _ = _points(refs, nd)
+    _lib.check(_lib.load().hd_nn_device(ctx.handle, int(index), nd, pq, nq, pr, nr,
+                                        C.c_void_p(out_d2.data_ptr()),
+                                        C.c_void_p(out_idx.data_ptr())))
+
+
+class HausdorffDistance:
+    """Mirror of hd::HausdorffDistance<COORD_T, N_DIMS>
+    (src/hausdorff_distance.h:15-42): calculate_distance returns sqrt(cmax2)
+    as a float and fills get_stats() with Algorithm / Execution /
+    ReportedTime. Inputs are torch CUDA tensors; unlike the reference they are
+    not shuffled in place."""
+
+    variant = Variant.EB
+    algorithm = "Early Break"
+
+    def __init__(self, ctx: Optional[Context] = None, seed: int = 0, **_: Any):
+        self.ctx = ctx or Context(0)
+        self.seed = seed
+        self._stats: dict = {}
+        self.last: Optional[DirectedResult] = None
+
+    def calculate_distance(self, points_a, points_b) -> float:
+        r = directed_device(self.ctx, self.variant, points_a, points_b, seed=self.seed)
+        self.last = r
+        self._stats = {"Algorithm": self.algorithm, "Execution": "GPU",
+                       "ReportedTime": r.time_ms, "ComparedPairs": r.evals,
+                       "CMax2Bits": r.cmax2_bits, "ArgMax": r.argmax, "NN": r.nn}
+        return r.hd
+
+    def get_stats(self) -> dict:
+        for k in ("Algorithm", "Execution", "ReportedTime"):
+            if k not in self._stats:
+                raise HDError(_lib.HD_EINVAL, f"stats lack {k}: call calculate_distance first")
+        return self._stats
+
+
+class HausdorffDistanceEarlyBreak(HausdorffDistance):
+    variant = Variant.EB
+    algorithm = "Early Break"
+
+
+class HausdorffDistanceGrid(HausdorffDistance):
+    variant = Variant.GRID
+    algorithm = "Uniform Grid"
+
+
+class HausdorffDistanceLbvh(HausdorffDistance):
+    variant = Variant.LBVH
+    algorithm = "LBVH"
+
+
+class HausdorffDistanceHybrid(HausdorffDistance):
+    variant = Variant.HYBRID
+    algorithm = "Hybrid"
+
+
+def device_count() -> int:
+    return int(_lib.load().hd_device_count())
+
+
+def device_info(device: int = 0) -> dict:
+    buf = C.create_string_buffer(4096)
+    _lib.check(_lib.load().hd_device_info(device, buf, 4096))
+    return json.loads(buf.value.decode())

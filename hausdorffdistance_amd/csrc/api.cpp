@@ -1,0 +1,601 @@
+// C ABI of libhd.so (include/hd.h): argument validation, contexts, the
+// RunHausdorffDistance driver with the reference's stats JSON, and dispatch
+// to the variant pipelines.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "hd.h"
+#include "hd_internal.h"
+#include "hd_primitives.h"
+
+namespace hd {
+
+namespace {
+thread_local std::string g_last_error;
+
+const char* AlgorithmName(int variant) {
+  switch (variant) {
+    case HD_VARIANT_EB:
+      return "Early Break";
+    case HD_VARIANT_GRID:
+      return "Uniform Grid";
+    case HD_VARIANT_LBVH:
+      return "LBVH";
+    case HD_VARIANT_HYBRID:
+      return "Hybrid";
+  }
+  return "Unknown";
+}
+
+std::string JsonEscape(const std::string& s) {
+  std::string o;
+  for (char c : s) {
+    if (c == '"' || c == '\\') {
+      o.push_back('\\');
+      o.push_back(c);
+    } else if ((unsigned char)c < 0x20) {
+      char buf[8];
+      snprintf(buf, sizeof(buf), "\\u%04x", c);
+      o += buf;
+    } else {
+      o.push_back(c);
+    }
+  }
+  return o;
+}
+
+std::string Num(double v) {
+  if (!std::isfinite(v)) return "null";
+  char buf[64];
+  snprintf(buf, sizeof(buf), "%.9g", v);
+  return buf;
+}
+
+struct ScopedContext {
+  hd_context* ctx = nullptr;
+  bool owned = false;
+  ~ScopedContext() {
+    if (owned) hd_context_destroy(ctx);
+  }
+};
+
+int AcquireContext(hd_context* given, ScopedContext* sc) {
+  if (given) {
+    sc->ctx = given;
+    return ContextActivate(given);
+  }
+  hd_context* c = nullptr;
+  int rc = hd_context_create(0, &c);
+  if (rc != HD_OK) return rc;
+  sc->ctx = c;
+  sc->owned = true;
+  return HD_OK;
+}
+
+bool ValidVariant(int v) { return v >= HD_VARIANT_EB && v <= HD_VARIANT_HYBRID; }
+
+int ValidateSizes(uint64_t n_a, uint64_t n_b) {
+  if (n_a == 0 || n_b == 0)
+    return Fail(HD_EINVAL, "empty point set (|A|=%llu, |B|=%llu)",
+                (unsigned long long)n_a, (unsigned long long)n_b);
+  if (n_a > 0xFFFFFFFEull || n_b > 0xFFFFFFFEull)
+    return Fail(HD_EINVAL, "point sets are limited to 2^32-2 points");
+  return HD_OK;
+}
+
+int Dispatch(hd_context* ctx, int variant, const DirectedArgs& args,
+             hd_directed_result* out) {
+  std::memset(out, 0, sizeof(*out));
+  out->variant_used = variant;
+  switch (variant) {
+    case HD_VARIANT_EB:
+      return RunEarlyBreak(ctx, args, out);
+    case HD_VARIANT_GRID:
+      return RunGrid(ctx, args, out);
+    case HD_VARIANT_LBVH:
+      return RunLbvh(ctx, args, out);
+    case HD_VARIANT_HYBRID:
+      return RunHybrid(ctx, args, out);
+  }
+  return Fail(HD_EINVAL, "unknown variant %d", variant);
+}
+
+// Timed directed run on device inputs.
+int TimedDirected(hd_context* ctx, int variant, const DirectedArgs& args,
+                  hd_directed_result* out) {
+  HD_HIP_TRY(hipEventRecord(ctx->ev_begin, ctx->stream));
+  HD_TRY(Dispatch(ctx, variant, args, out));
+  HD_HIP_TRY(hipEventRecord(ctx->ev_end, ctx->stream));
+  HD_HIP_TRY(hipEventSynchronize(ctx->ev_end));
+  float ms = 0.f;
+  HD_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
+  out->time_ms = ms;
+  return HD_OK;
+}
+
+}  // namespace
+
+void SetError(const std::string& msg) { g_last_error = msg; }
+void ClearError() { g_last_error.clear(); }
+
+int Fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int ContextActivate(hd_context* ctx) {
+  HD_HIP_TRY(hipSetDevice(ctx->device));
+  return HD_OK;
+}
+
+int ContextBuffer(hd_context* ctx, const char* name, size_t bytes,
+                  void** out) {
+  DeviceBuffer& b = ctx->buffers[name];
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      // The stream may still use the old buffer.
+      HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+      HD_HIP_TRY(hipFree(b.ptr));
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess) {
+      b.ptr = nullptr;
+      (void)hipGetLastError();
+      return Fail(HD_ENOMEM, "hipMalloc(%zu) for %s failed: %s", want, name,
+                  hipGetErrorString(e));
+    }
+    b.bytes = want;
+  }
+  *out = b.ptr;
+  return HD_OK;
+}
+
+int ContextPinned(hd_context* ctx, size_t bytes, void** out) {
+  if (ctx->host_pinned_bytes < bytes) {
+    if (ctx->host_pinned) {
+      HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+      HD_HIP_TRY(hipHostFree(ctx->host_pinned));
+      ctx->host_pinned = nullptr;
+    }
+    size_t want = std::max<size_t>(bytes, 4096);
+    HD_HIP_TRY(hipHostMalloc(&ctx->host_pinned, want, hipHostMallocDefault));
+    ctx->host_pinned_bytes = want;
+  }
+  *out = ctx->host_pinned;
+  return HD_OK;
+}
+
+int StageBegin(hd_context* ctx, const char* name) {
+  if (!ctx->profiling) return -1;
+  PendingStage st;
+  st.name = name;
+  for (hipEvent_t* ev : {&st.begin, &st.end}) {
+    if (!ctx->event_pool.empty()) {
+      *ev = ctx->event_pool.back();
+      ctx->event_pool.pop_back();
+    } else if (hipEventCreate(ev) != hipSuccess) {
+      (void)hipGetLastError();
+      return -1;
+    }
+  }
+  if (hipEventRecord(st.begin, ctx->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->event_pool.push_back(st.begin);
+    ctx->event_pool.push_back(st.end);
+    return -1;
+  }
+  ctx->pending_stages.push_back(st);
+  return (int)ctx->pending_stages.size() - 1;
+}
+
+void StageEnd(hd_context* ctx, int handle) {
+  if (handle < 0 || handle >= (int)ctx->pending_stages.size()) return;
+  if (hipEventRecord(ctx->pending_stages[handle].end, ctx->stream) !=
+      hipSuccess)
+    (void)hipGetLastError();
+}
+
+void StageCount(hd_context* ctx, const char* name, uint64_t evals,
+                uint64_t points) {
+  if (!ctx->profiling) return;
+  StageStat& s = ctx->stage_stats[name];
+  s.evals += evals;
+  s.points += points;
+}
+
+int ResolveStages(hd_context* ctx) {
+  if (ctx->pending_stages.empty()) return HD_OK;
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (PendingStage& st : ctx->pending_stages) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, st.begin, st.end) == hipSuccess) {
+      StageStat& s = ctx->stage_stats[st.name];
+      s.total_ms += ms;
+      s.calls += 1;
+    } else {
+      (void)hipGetLastError();
+    }
+    ctx->event_pool.push_back(st.begin);
+    ctx->event_pool.push_back(st.end);
+  }
+  ctx->pending_stages.clear();
+  return HD_OK;
+}
+
+}  // namespace hd
+
+using namespace hd;
+
+extern "C" {
+
+const char* hd_last_error(void) { return g_last_error.c_str(); }
+
+void hd_run_config_init(hd_run_config* cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->abi_version = HD_ABI_VERSION;
+  cfg->variant = HD_VARIANT_EB;
+  cfg->execution = HD_EXEC_GPU;
+  cfg->n_dims = 0;
+  cfg->repeat = 5;
+  cfg->seed = 0;
+  cfg->parallelism = -1;
+  cfg->flags = 0;
+  cfg->sample_rate = 0.01f;
+  cfg->n_points_cell = 8;
+  cfg->max_hit = 256;
+  cfg->eb_only_threshold = 0;
+  cfg->auto_tune = 0;
+  cfg->model = nullptr;
+}
+
+int hd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int hd_device_info(int device, char* buf, uint64_t cap) {
+  ClearError();
+  if (!buf || cap == 0) return Fail(HD_EINVAL, "null buffer");
+  hipDeviceProp_t prop;
+  HD_HIP_TRY(hipGetDeviceProperties(&prop, device));
+  std::ostringstream os;
+  os << "{\"Device\": " << device << ", \"name\": \"" << JsonEscape(prop.name)
+     << "\", \"gcnArchName\": \"" << JsonEscape(prop.gcnArchName)
+     << "\", \"multiProcessorCount\": " << prop.multiProcessorCount
+     << ", \"clockRateKHz\": " << prop.clockRate
+     << ", \"memoryClockRateKHz\": " << prop.memoryClockRate
+     << ", \"memoryBusWidth\": " << prop.memoryBusWidth
+     << ", \"totalGlobalMemBytes\": " << prop.totalGlobalMem << "}";
+  std::string s = os.str();
+  std::strncpy(buf, s.c_str(), cap - 1);
+  buf[cap - 1] = '\0';
+  return s.size() + 1 > cap ? HD_ETRUNC : HD_OK;
+}
+
+int hd_context_create(int device, hd_context** out) {
+  ClearError();
+  if (!out) return Fail(HD_EINVAL, "null output pointer");
+  *out = nullptr;
+  int n = hd_device_count();
+  if (n <= 0) return Fail(HD_EDEVICE, "no HIP device is visible");
+  if (device < 0 || device >= n)
+    return Fail(HD_EINVAL, "device %d out of range (have %d)", device, n);
+  std::unique_ptr<hd_context> ctx(new hd_context());
+  ctx->device = device;
+  HD_HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HD_HIP_TRY(hipGetDeviceProperties(&prop, device));
+  ctx->cu_count = prop.multiProcessorCount;
+  HD_HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HD_HIP_TRY(hipEventCreate(&ctx->ev_begin));
+  HD_HIP_TRY(hipEventCreate(&ctx->ev_end));
+  *out = ctx.release();
+  return HD_OK;
+}
+
+void hd_context_destroy(hd_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->buffers)
+    if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  for (auto& st : ctx->pending_stages) {
+    (void)hipEventDestroy(st.begin);
+    (void)hipEventDestroy(st.end);
+  }
+  for (hipEvent_t ev : ctx->event_pool) (void)hipEventDestroy(ev);
+  if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  if (ctx->ev_begin) (void)hipEventDestroy(ctx->ev_begin);
+  if (ctx->ev_end) (void)hipEventDestroy(ctx->ev_end);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int hd_context_set_profiling(hd_context* ctx, int enable) {
+  ClearError();
+  if (!ctx) return Fail(HD_EINVAL, "null context");
+  HD_TRY(ResolveStages(ctx));
+  ctx->profiling = enable != 0;
+  return HD_OK;
+}
+
+int hd_context_stage_times(hd_context* ctx, char* json, uint64_t cap,
+                           int reset) {
+  ClearError();
+  if (!ctx || !json || cap == 0) return Fail(HD_EINVAL, "null argument");
+  HD_TRY(ResolveStages(ctx));
+  std::ostringstream os;
+  os << "{";
+  bool first = true;
+  for (const auto& kv : ctx->stage_stats) {
+    os << (first ? "" : ", ") << "\"" << JsonEscape(kv.first)
+       << "\": {\"ms\": " << Num(kv.second.total_ms)
+       << ", \"calls\": " << kv.second.calls
+       << ", \"evals\": " << kv.second.evals
+       << ", \"points\": " << kv.second.points << "}";
+    first = false;
+  }
+  os << "}";
+  if (reset) ctx->stage_stats.clear();
+  const std::string s = os.str();
+  std::strncpy(json, s.c_str(), cap - 1);
+  json[cap - 1] = '\0';
+  return s.size() + 1 > cap ? Fail(HD_ETRUNC, "stage JSON truncated") : HD_OK;
+}
+
+void* hd_context_stream(hd_context* ctx) {
+  return ctx ? static_cast<void*>(ctx->stream) : nullptr;
+}
+
+int hd_directed_device(hd_context* ctx, int variant, int n_dims,
+                       const float* d_a, uint64_t n_a, const float* d_b,
+                       uint64_t n_b, uint32_t flags, int32_t seed,
+                       const hd_mlp* model, hd_directed_result* out) {
+  ClearError();
+  if (!out) return Fail(HD_EINVAL, "null result pointer");
+  std::memset(out, 0, sizeof(*out));
+  if (n_dims != 2 && n_dims != 3)
+    return Fail(HD_EINVAL, "n_dims must be 2 or 3, got %d", n_dims);
+  if (!ValidVariant(variant))
+    return Fail(HD_EINVAL, "unknown variant %d", variant);
+  HD_TRY(ValidateSizes(n_a, n_b));
+  if (!d_a || !d_b) return Fail(HD_EINVAL, "null point array");
+  ScopedContext sc;
+  HD_TRY(AcquireContext(ctx, &sc));
+  float mbr_a[6], mbr_b[6];
+  HD_TRY(HostMbr(sc.ctx, n_dims, d_a, n_a, mbr_a));
+  HD_TRY(HostMbr(sc.ctx, n_dims, d_b, n_b, mbr_b));
+  DirectedArgs args;
+  args.n_dims = n_dims;
+  args.a = d_a;
+  args.n_a = n_a;
+  args.b = d_b;
+  args.n_b = n_b;
+  args.flags = flags;
+  args.seed = seed;
+  args.model = model;
+  return TimedDirected(sc.ctx, variant, args, out);
+}
+
+int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
+                              const float* a, uint64_t n_a, const float* b,
+                              uint64_t n_b, hd_result* out, char* stats_json,
+                              uint64_t stats_cap) {
+  ClearError();
+  if (!cfg || !out) return Fail(HD_EINVAL, "null config or result");
+  std::memset(out, 0, sizeof(*out));
+  out->check_pass = -1;
+  if (cfg->abi_version != HD_ABI_VERSION)
+    return Fail(HD_EINVAL, "abi_version %u, library speaks %u",
+                cfg->abi_version, HD_ABI_VERSION);
+  if (cfg->n_dims != 2 && cfg->n_dims != 3)
+    return Fail(HD_EINVAL,
+                "Wrong number of dimensions, which can only be 2 or 3 (got %d)",
+                cfg->n_dims);
+  if (!ValidVariant(cfg->variant))
+    return Fail(HD_EINVAL, "Unknown variant: %d", cfg->variant);
+  if (cfg->execution != HD_EXEC_GPU && cfg->execution != HD_EXEC_CPU)
+    return Fail(HD_EINVAL, "Unknown execution: %d", cfg->execution);
+  if (cfg->execution == HD_EXEC_CPU)
+    return Fail(HD_EUNSUPPORTED,
+                "CPU execution is not served by this library (GPU only)");
+  if (cfg->repeat < 1)
+    return Fail(HD_EINVAL, "repeat must be >= 1, got %d", cfg->repeat);
+  HD_TRY(ValidateSizes(n_a, n_b));
+  if (!a || !b) return Fail(HD_EINVAL, "null point array");
+  const int nd = cfg->n_dims;
+
+  ScopedContext sc;
+  HD_TRY(AcquireContext(ctx, &sc));
+  hd_context* c = sc.ctx;
+
+  // Upload once, before the repeats (reference run_hausdorff_distance.cu:148).
+  const float* d_a = a;
+  const float* d_b = b;
+  if (!(cfg->flags & HD_FLAG_INPUT_ON_DEVICE)) {
+    float *ua = nullptr, *ub = nullptr;
+    HD_TRY(ContextArray(c, "input_a", n_a * nd, &ua));
+    HD_TRY(ContextArray(c, "input_b", n_b * nd, &ub));
+    HD_HIP_TRY(hipMemcpyAsync(ua, a, n_a * nd * sizeof(float),
+                              hipMemcpyHostToDevice, c->stream));
+    HD_HIP_TRY(hipMemcpyAsync(ub, b, n_b * nd * sizeof(float),
+                              hipMemcpyHostToDevice, c->stream));
+    d_a = ua;
+    d_b = ub;
+  }
+  float mbr_a[6], mbr_b[6];
+  HD_TRY(HostMbr(c, nd, d_a, n_a, mbr_a));
+  HD_TRY(HostMbr(c, nd, d_b, n_b, mbr_b));
+
+  DirectedArgs ab;
+  ab.n_dims = nd;
+  ab.a = d_a;
+  ab.n_a = n_a;
+  ab.b = d_b;
+  ab.n_b = n_b;
+  ab.flags = cfg->flags;
+  ab.seed = cfg->seed;
+  ab.model = static_cast<const hd_mlp*>(cfg->model);
+  ab.n_points_cell = cfg->n_points_cell;
+  ab.max_hit = cfg->max_hit;
+  ab.sample_rate = cfg->sample_rate;
+  ab.eb_only_threshold = cfg->eb_only_threshold;
+  ab.auto_tune = cfg->auto_tune;
+  DirectedArgs ba = ab;
+  std::swap(ba.a, ba.b);
+  std::swap(ba.n_a, ba.n_b);
+  const bool symmetric = (cfg->flags & HD_FLAG_SYMMETRIC) != 0;
+
+  std::vector<double> times;
+  std::ostringstream reps;
+  for (int r = 0; r < cfg->repeat; ++r) {
+    hd_directed_result rab, rba;
+    std::memset(&rba, 0, sizeof(rba));
+    HD_TRY(TimedDirected(c, cfg->variant, ab, &rab));
+    double t = rab.time_ms;
+    if (symmetric) {
+      HD_TRY(TimedDirected(c, cfg->variant, ba, &rba));
+      t += rba.time_ms;
+    }
+    times.push_back(t);
+    out->ab = rab;
+    out->ba = rba;
+    if (r) reps << ", ";
+    reps << "{\"Repeat\": " << (r + 1) << ", \"Algorithm\": \""
+         << AlgorithmName(rab.variant_used)
+         << "\", \"Execution\": \"GPU\", \"ReportedTime\": " << Num(t)
+         << ", \"ComparedPairs\": " << (rab.evals + rba.evals)
+         << ", \"CMax2Bits\": " << rab.cmax2_bits
+         << ", \"ArgMax\": " << rab.argmax << "}";
+  }
+  // Result.
+  out->n_repeats = (uint32_t)cfg->repeat;
+  const bool ba_wins = symmetric && out->ba.cmax2_bits > out->ab.cmax2_bits;
+  const hd_directed_result& win = ba_wins ? out->ba : out->ab;
+  out->hd = win.hd;
+  out->cmax2_bits = win.cmax2_bits;
+  out->argmax = win.argmax;
+  out->nn = win.nn;
+  out->direction = ba_wins ? 1 : 0;
+  double sum = 0;
+  for (double t : times) sum += t;
+  std::vector<double> sorted = times;
+  std::sort(sorted.begin(), sorted.end());
+  out->avg_time_ms = sum / times.size();
+  out->min_time_ms = sorted.front();
+  out->median_time_ms = sorted[sorted.size() / 2];
+
+  // Check (reference run_hausdorff_distance.cu:366-388 re-runs the CPU EB and
+  // compares with ==; here the plain GPU EB is the independent path).
+  std::string check_json;
+  if (cfg->flags & HD_FLAG_CHECK) {
+    DirectedArgs cab = ab, cba = ba;
+    cab.flags &= ~(HD_FLAG_LEARNED_ORDER);
+    cba.flags &= ~(HD_FLAG_LEARNED_ORDER);
+    hd_directed_result rab, rba;
+    std::memset(&rba, 0, sizeof(rba));
+    HD_TRY(Dispatch(c, HD_VARIANT_EB, cab, &rab));
+    if (symmetric) HD_TRY(Dispatch(c, HD_VARIANT_EB, cba, &rba));
+    const bool pass = rab.cmax2_bits == out->ab.cmax2_bits &&
+                      rab.argmax == out->ab.argmax &&
+                      (!symmetric || (rba.cmax2_bits == out->ba.cmax2_bits &&
+                                      rba.argmax == out->ba.argmax));
+    const hd_directed_result& cw =
+        (symmetric && rba.cmax2_bits > rab.cmax2_bits) ? rba : rab;
+    out->check_pass = pass ? 1 : 0;
+    check_json = ", \"Check\": {\"HDAnswer\": " + Num(cw.hd) +
+                 ", \"Diff\": " + Num((double)cw.hd - (double)out->hd) +
+                 ", \"Pass\": " + (pass ? "true" : "false") + "}";
+  }
+
+  if (stats_json && stats_cap > 0) {
+    hipDeviceProp_t prop;
+    HD_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
+    auto mbr_json = [&](const float* m) {
+      std::ostringstream o;
+      o << "[";
+      for (int d = 0; d < nd; ++d)
+        o << (d ? ", " : "") << "{\"Lower\": " << Num(m[d])
+          << ", \"Upper\": " << Num(m[nd + d]) << "}";
+      o << "]";
+      return o.str();
+    };
+    std::ostringstream js;
+    js << "{\"GPU\": {\"Device\": " << c->device << ", \"name\": \""
+       << JsonEscape(prop.name) << "\"}, \"Input\": {\"Files\": [{"
+       << "\"NumPoints\": " << n_a << ", \"MBR\": " << mbr_json(mbr_a)
+       << "}, {\"NumPoints\": " << n_b << ", \"MBR\": " << mbr_json(mbr_b)
+       << "}], \"NumDims\": " << nd << ", \"Type\": \"Float\"}"
+       << ", \"Running\": {\"Seed\": " << cfg->seed
+       << ", \"SampleRate\": " << Num(cfg->sample_rate)
+       << ", \"NumPointsPerCell\": " << cfg->n_points_cell
+       << ", \"MaxHit\": " << cfg->max_hit
+       << ", \"EBOnlyThreshold\": " << cfg->eb_only_threshold
+       << ", \"AutoTune\": " << (cfg->auto_tune ? "true" : "false")
+       << ", \"Symmetric\": " << (symmetric ? "true" : "false")
+       << ", \"LearnedOrder\": "
+       << ((cfg->flags & HD_FLAG_LEARNED_ORDER) ? "true" : "false")
+       << ", \"Repeats\": [" << reps.str()
+       << "], \"AvgTime\": " << Num(out->avg_time_ms)
+       << ", \"MedianTime\": " << Num(out->median_time_ms) << "}"
+       << ", \"HDResult\": " << Num(out->hd)
+       << ", \"CMax2Bits\": " << out->cmax2_bits
+       << ", \"ArgMax\": " << out->argmax << ", \"NN\": " << out->nn
+       << ", \"Direction\": " << out->direction << check_json << "}";
+    const std::string s = js.str();
+    std::strncpy(stats_json, s.c_str(), stats_cap - 1);
+    stats_json[stats_cap - 1] = '\0';
+    if (s.size() + 1 > stats_cap) {
+      Fail(HD_ETRUNC, "stats JSON needs %zu bytes", s.size() + 1);
+      return HD_ETRUNC;
+    }
+  }
+  return HD_OK;
+}
+
+int hd_nn_device(hd_context* ctx, int index, int n_dims, const float* d_q,
+                 uint64_t n_q, const float* d_ref, uint64_t n_ref,
+                 float* d_out_d2, uint32_t* d_out_idx) {
+  ClearError();
+  if (n_dims != 2 && n_dims != 3)
+    return Fail(HD_EINVAL, "n_dims must be 2 or 3, got %d", n_dims);
+  if (index != HD_VARIANT_EB && index != HD_VARIANT_GRID &&
+      index != HD_VARIANT_LBVH)
+    return Fail(HD_EINVAL, "index must be EB, GRID or LBVH, got %d", index);
+  HD_TRY(ValidateSizes(n_q, n_ref));
+  if (!d_q || !d_ref || !d_out_d2 || !d_out_idx)
+    return Fail(HD_EINVAL, "null pointer");
+  ScopedContext sc;
+  HD_TRY(AcquireContext(ctx, &sc));
+  float mbr[6];
+  HD_TRY(HostMbr(sc.ctx, n_dims, d_q, n_q, mbr));
+  HD_TRY(HostMbr(sc.ctx, n_dims, d_ref, n_ref, mbr));
+  HD_TRY(NearestNeighbors(sc.ctx, index, n_dims, d_q, n_q, d_ref, n_ref,
+                          d_out_d2, d_out_idx));
+  HD_HIP_TRY(hipStreamSynchronize(sc.ctx->stream));
+  return HD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,277 @@
+// Support kernels shared by every variant: fills, AoS -> SoA repacking with
+// an optional pseudo-random permutation, MBR reduction with input
+// validation, and the post-pass that names the arg-max's nearest neighbour.
+#include <cmath>
+#include <cstring>
+
+#include "hd.h"
+#include "hd_device.h"
+#include "hd_gen.h"
+#include "hd_internal.h"
+#include "hd_primitives.h"
+
+namespace hd {
+
+using namespace dev;
+
+namespace {
+
+__global__ void FillU32Kernel(uint32_t* p, uint32_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void FillU64Kernel(unsigned long long* p, unsigned long long v,
+                              uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+template <int ND>
+__global__ void GatherSoAKernel(const float* __restrict__ aos, uint32_t n,
+                                uint32_t n_pad, int shuffle, Feistel perm,
+                                float* __restrict__ x, float* __restrict__ y,
+                                float* __restrict__ z) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_pad;
+       j += gridDim.x * blockDim.x) {
+    uint32_t src = j < n ? j : j % n;
+    if (shuffle) src = Permute(perm, src);
+    const float* p = aos + (uint64_t)src * ND;
+    x[j] = p[0];
+    y[j] = p[1];
+    if constexpr (ND == 3) z[j] = p[2];
+  }
+}
+
+// Order-preserving map of FP32 to uint32 (negative values reversed).
+HD_DEV uint32_t OrderedBits(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Per block: min/max per coordinate and a count of non-finite coordinates.
+// out[0..ND) = ordered min, out[ND..2ND) = ordered max, out[2ND] = bad count.
+template <int ND>
+__global__ __launch_bounds__(256) void MbrKernel(const float* __restrict__ aos,
+                                                 uint64_t n,
+                                                 uint32_t* __restrict__ out) {
+  uint32_t lo[ND], hi[ND];
+  uint32_t bad = 0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    lo[d] = 0xFFFFFFFFu;
+    hi[d] = 0u;
+  }
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      float v = aos[i * ND + d];
+      if (!isfinite(v)) {
+        bad++;
+        continue;
+      }
+      uint32_t k = OrderedBits(v);
+      lo[d] = min(lo[d], k);
+      hi[d] = max(hi[d], k);
+    }
+  }
+  __shared__ uint32_t s_lo[4][ND], s_hi[4][ND], s_bad[4];
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    lo[d] = WaveMinU32(lo[d]);
+    uint32_t h = hi[d];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      h = max(h, (uint32_t)__shfl_xor((int)h, off, 64));
+    hi[d] = h;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) bad += __shfl_xor((int)bad, off, 64);
+  if (LaneId() == 0) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      s_lo[wave][d] = lo[d];
+      s_hi[wave][d] = hi[d];
+    }
+    s_bad[wave] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int w = 1; w < nw; ++w) {
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        lo[d] = min(lo[d], s_lo[w][d]);
+        hi[d] = max(hi[d], s_hi[w][d]);
+      }
+      bad += s_bad[w];
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      atomicMin(&out[d], lo[d]);
+      atomicMax(&out[ND + d], hi[d]);
+    }
+    if (bad) atomicAdd(&out[2 * ND], bad);
+  }
+}
+
+float FromOrderedBits(uint32_t k) {
+  uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+template <int ND>
+__global__ void NnIndexKernel(const float* __restrict__ a, uint32_t argmax,
+                              const float* __restrict__ b, uint64_t n_b,
+                              uint32_t target_bits, uint32_t* out) {
+  float pa[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) pa[d] = a[(uint64_t)argmax * ND + d];
+  uint32_t best = 0xFFFFFFFFu;
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n_b;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const float* pb = b + j * ND;
+    float d2 = ND == 2 ? D2(pa[0], pa[1], pb[0], pb[1])
+                       : D2(pa[0], pa[1], pa[2], pb[0], pb[1], pb[2]);
+    if (__float_as_uint(d2) == target_bits) {
+      best = (uint32_t)j;
+      break;  // j only grows along this thread's stride
+    }
+  }
+  best = WaveMinU32(best);
+  if (LaneId() == 0 && best != 0xFFFFFFFFu) atomicMin(out, best);
+}
+
+inline int GridFor(uint64_t n, int threads, int cap) {
+  uint64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > (uint64_t)cap) b = cap;
+  return (int)b;
+}
+
+}  // namespace
+
+Feistel MakeFeistel(uint32_t n, uint64_t seed, uint32_t stream_id) {
+  Feistel f;
+  f.n = n ? n : 1;
+  uint32_t bits = 1;
+  while (bits < 32 && (1ull << bits) < f.n) ++bits;
+  f.half = (bits + 1) / 2;
+  if (f.half < 1) f.half = 1;
+  f.mask = (1u << f.half) - 1u;
+  for (int i = 0; i < 4; ++i)
+    f.key[i] = (uint32_t)hd_splitmix64_at(seed, stream_id * 4u + i);
+  return f;
+}
+
+int FillU32(hd_context* ctx, uint32_t* p, uint32_t value, uint64_t n) {
+  if (n == 0) return HD_OK;
+  FillU32Kernel<<<GridFor(n, 256, 4096), 256, 0, ctx->stream>>>(p, value, n);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
+
+int FillU64(hd_context* ctx, unsigned long long* p, unsigned long long value,
+            uint64_t n) {
+  if (n == 0) return HD_OK;
+  FillU64Kernel<<<GridFor(n, 256, 4096), 256, 0, ctx->stream>>>(p, value, n);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
+
+int GatherSoA(hd_context* ctx, int n_dims, const float* aos, uint32_t n,
+              uint32_t n_pad, bool shuffle, const Feistel& perm, float* x,
+              float* y, float* z) {
+  if (n == 0 || n_pad < n) return Fail(HD_EINVAL, "GatherSoA: bad sizes");
+  const int grid = GridFor(n_pad, 256, 8192);
+  if (n_dims == 2)
+    GatherSoAKernel<2><<<grid, 256, 0, ctx->stream>>>(aos, n, n_pad, shuffle,
+                                                      perm, x, y, z);
+  else
+    GatherSoAKernel<3><<<grid, 256, 0, ctx->stream>>>(aos, n, n_pad, shuffle,
+                                                      perm, x, y, z);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
+
+int HostMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
+            float* mbr) {
+  if (n == 0) return Fail(HD_EINVAL, "empty point set");
+  uint32_t* d_out = nullptr;
+  HD_TRY(ContextArray(ctx, "mbr_keys", 8, &d_out));
+  // layout: lo[0..nd), hi[nd..2nd), non-finite count at [2nd]
+  HD_TRY(FillU32(ctx, d_out, 0xFFFFFFFFu, n_dims));
+  HD_TRY(FillU32(ctx, d_out + n_dims, 0u, n_dims + 1));
+  const int grid = GridFor(n, 256, ctx->cu_count * 4);
+  if (n_dims == 2)
+    MbrKernel<2><<<grid, 256, 0, ctx->stream>>>(aos, n, d_out);
+  else
+    MbrKernel<3><<<grid, 256, 0, ctx->stream>>>(aos, n, d_out);
+  HD_HIP_TRY(hipGetLastError());
+  uint32_t h[8];
+  HD_HIP_TRY(hipMemcpyAsync(h, d_out, sizeof(uint32_t) * (2 * n_dims + 1),
+                            hipMemcpyDeviceToHost, ctx->stream));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (h[2 * n_dims] != 0)
+    return Fail(HD_EINVAL, "%u non-finite coordinates in input",
+                h[2 * n_dims]);
+  for (int d = 0; d < 2 * n_dims; ++d) mbr[d] = FromOrderedBits(h[d]);
+  for (int d = 0; d < n_dims; ++d)
+    if (mbr[n_dims + d] - mbr[d] == 0.0f) mbr[n_dims + d] = mbr[d] + 1e-5f;
+  return HD_OK;
+}
+
+int FinishDirected(hd_context* ctx, int n_dims, const float* a,
+                   const float* b, uint64_t n_b,
+                   const unsigned long long* key_dev,
+                   const unsigned long long* evals_dev, int n_counters,
+                   hd_directed_result* out) {
+  if (n_counters > 8) return Fail(HD_EINVAL, "too many counters");
+  void* pinned = nullptr;
+  HD_TRY(ContextPinned(ctx, 128, &pinned));
+  unsigned long long* h = static_cast<unsigned long long*>(pinned);
+  HD_HIP_TRY(hipMemcpyAsync(&h[0], key_dev, 8, hipMemcpyDeviceToHost,
+                            ctx->stream));
+  if (evals_dev && n_counters > 0)
+    HD_HIP_TRY(hipMemcpyAsync(&h[1], evals_dev, 8 * n_counters,
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const unsigned long long key = h[0];
+  out->evals = 0;
+  if (evals_dev)
+    for (int i = 0; i < n_counters; ++i) out->evals += h[1 + i];
+  out->cmax2_bits = (uint32_t)(key >> 32);
+  float c;
+  std::memcpy(&c, &out->cmax2_bits, 4);
+  out->hd = std::sqrt(c);
+  if (key == 0) {
+    out->argmax = 0xFFFFFFFFu;
+    out->nn = 0xFFFFFFFFu;
+    return HD_OK;
+  }
+  out->argmax = 0xFFFFFFFFu - (uint32_t)key;
+  uint32_t* d_nn = nullptr;
+  HD_TRY(ContextArray(ctx, "nn_index", 1, &d_nn));
+  HD_TRY(FillU32(ctx, d_nn, 0xFFFFFFFFu, 1));
+  const int grid = GridFor(n_b, 256, ctx->cu_count * 8);
+  if (n_dims == 2)
+    NnIndexKernel<2><<<grid, 256, 0, ctx->stream>>>(a, out->argmax, b, n_b,
+                                                    out->cmax2_bits, d_nn);
+  else
+    NnIndexKernel<3><<<grid, 256, 0, ctx->stream>>>(a, out->argmax, b, n_b,
+                                                    out->cmax2_bits, d_nn);
+  HD_HIP_TRY(hipGetLastError());
+  uint32_t* hn = reinterpret_cast<uint32_t*>(&h[12]);
+  HD_HIP_TRY(hipMemcpyAsync(hn, d_nn, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  out->nn = *hn;
+  return HD_OK;
+}
+
+}  // namespace hd

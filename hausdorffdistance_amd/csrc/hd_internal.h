@@ -1,0 +1,179 @@
+// Host-side internals shared by the library's translation units.
+#ifndef HD_INTERNAL_H
+#define HD_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "hd.h"
+
+namespace hd {
+
+// Thread-local error message behind hd_last_error().
+void SetError(const std::string& msg);
+void ClearError();
+
+// Formats and records an error, returns `code`.
+int Fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define HD_HIP_TRY(expr)                                                    \
+  do {                                                                      \
+    hipError_t hd_err_ = (expr);                                            \
+    if (hd_err_ != hipSuccess) {                                            \
+      return ::hd::Fail(hd_err_ == hipErrorOutOfMemory ? HD_ENOMEM          \
+                                                       : HD_EDEVICE,        \
+                        "%s failed: %s (%s:%d)", #expr,                     \
+                        hipGetErrorString(hd_err_), __FILE__, __LINE__);    \
+    }                                                                       \
+  } while (0)
+
+#define HD_TRY(expr)             \
+  do {                           \
+    int hd_rc_ = (expr);         \
+    if (hd_rc_ != HD_OK) return hd_rc_; \
+  } while (0)
+
+// Parameters of a bijective pseudo-random permutation of [0, n) evaluated on
+// the device (hd_device.h: Permute).
+struct Feistel {
+  uint32_t n;
+  uint32_t half;
+  uint32_t mask;
+  uint32_t key[4];
+};
+namespace dev {
+using hd::Feistel;
+}
+
+// Grow-only device buffer cached in a context under a name.
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Per-stage device timing (HIP events on the context stream), accumulated
+// across calls while profiling is on.
+struct StageStat {
+  double total_ms = 0;
+  uint64_t calls = 0;
+  uint64_t evals = 0;
+  uint64_t points = 0;
+};
+struct PendingStage {
+  std::string name;
+  hipEvent_t begin, end;
+};
+
+}  // namespace hd
+
+// Opaque handle of the C ABI.
+struct hd_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  int max_waves_per_cu = 32;
+  std::unordered_map<std::string, hd::DeviceBuffer> buffers;
+  void* host_pinned = nullptr;  // small pinned staging area
+  size_t host_pinned_bytes = 0;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  bool profiling = false;
+  std::vector<hipEvent_t> event_pool;
+  std::vector<hd::PendingStage> pending_stages;
+  std::map<std::string, hd::StageStat> stage_stats;
+};
+
+namespace hd {
+
+// Returns a device buffer of at least `bytes` under `name` (contents are
+// undefined; the buffer is reused across calls on the same context).
+int ContextBuffer(hd_context* ctx, const char* name, size_t bytes, void** out);
+
+template <typename T>
+int ContextArray(hd_context* ctx, const char* name, size_t count, T** out) {
+  void* p = nullptr;
+  int rc = ContextBuffer(ctx, name, count * sizeof(T) + 256, &p);
+  *out = static_cast<T*>(p);
+  return rc;
+}
+
+// Pinned host staging of at least `bytes`.
+int ContextPinned(hd_context* ctx, size_t bytes, void** out);
+
+// Makes ctx->device current on the calling thread.
+int ContextActivate(hd_context* ctx);
+
+// Stage timing: StageBegin records a start event when profiling is on and
+// returns a handle (-1 when off); StageEnd records the end event.
+// StageCount adds work counters to a stage. Pending events are resolved
+// lazily by ResolveStages (after the stream is synchronised).
+int StageBegin(hd_context* ctx, const char* name);
+void StageEnd(hd_context* ctx, int handle);
+void StageCount(hd_context* ctx, const char* name, uint64_t evals,
+                uint64_t points);
+int ResolveStages(hd_context* ctx);
+
+// Persistent grid size for wave-level kernels of `threads_per_block`.
+inline int PersistentBlocks(const hd_context* ctx, int threads_per_block,
+                            int waves_per_simd) {
+  int waves_per_block = threads_per_block / 64;
+  int waves = ctx->cu_count * 4 * waves_per_simd;
+  int blocks = waves / waves_per_block;
+  return blocks > 0 ? blocks : 1;
+}
+
+// Device-side HD pipelines (implemented in the .hip files).
+struct DirectedArgs {
+  int n_dims = 2;
+  const float* a = nullptr;  // AoS device pointer
+  uint64_t n_a = 0;
+  const float* b = nullptr;  // AoS device pointer
+  uint64_t n_b = 0;
+  uint32_t flags = 0;
+  int32_t seed = 0;
+  const hd_mlp* model = nullptr;
+  // tuning knobs (0 = library default)
+  uint32_t n_points_cell = 0;
+  uint32_t max_hit = 0;
+  uint32_t slot_budget_steps = 0;
+  uint32_t seed_points = 0;
+  float sample_rate = 0.f;
+  uint32_t eb_only_threshold = 0;
+  int auto_tune = 0;
+};
+
+int RunEarlyBreak(hd_context* ctx, const DirectedArgs& args,
+                  hd_directed_result* out);
+int RunGrid(hd_context* ctx, const DirectedArgs& args,
+            hd_directed_result* out);
+int RunLbvh(hd_context* ctx, const DirectedArgs& args,
+            hd_directed_result* out);
+int RunHybrid(hd_context* ctx, const DirectedArgs& args,
+              hd_directed_result* out);
+
+// Exact NN of queries against ref by index kind.
+int NearestNeighbors(hd_context* ctx, int index, int n_dims, const float* q,
+                     uint64_t n_q, const float* ref, uint64_t n_ref,
+                     float* out_d2, uint32_t* out_idx);
+
+// Builds the A visit order in descending predicted NN distance.
+int LearnedOrder(hd_context* ctx, const DirectedArgs& args,
+                 const uint32_t** order_out);
+
+// Finishes a directed result from the device key: reads it and the
+// n_counters evaluation counters at evals_dev (summed), and finds the NN
+// index of the arg-max.
+int FinishDirected(hd_context* ctx, int n_dims, const float* a,
+                   const float* b, uint64_t n_b,
+                   const unsigned long long* key_dev,
+                   const unsigned long long* evals_dev, int n_counters,
+                   hd_directed_result* out);
+
+}  // namespace hd
+
+#endif  // HD_INTERNAL_H

@@ -1,0 +1,50 @@
+// Host entry points of the device-wide support primitives (sort.hip,
+// common.hip). All run on the context stream and are asynchronous.
+#ifndef HD_PRIMITIVES_H
+#define HD_PRIMITIVES_H
+
+#include <cstdint>
+
+#include "hd_internal.h"
+
+namespace hd {
+
+int SortPairsDescF32(hd_context* ctx, const float* k_in, float* k_out,
+                     const uint32_t* v_in, uint32_t* v_out, uint32_t n);
+int SortPairsU32(hd_context* ctx, const uint32_t* k_in, uint32_t* k_out,
+                 const uint32_t* v_in, uint32_t* v_out, uint32_t n,
+                 int end_bit);
+int SortPairsU64(hd_context* ctx, const unsigned long long* k_in,
+                 unsigned long long* k_out, const uint32_t* v_in,
+                 uint32_t* v_out, uint32_t n, int end_bit);
+int ExclusiveScanU32(hd_context* ctx, const uint32_t* in, uint32_t* out,
+                     uint32_t n);
+
+// Host construction of a Feistel permutation of [0, n) (hd_device.h).
+Feistel MakeFeistel(uint32_t n, uint64_t seed, uint32_t stream_id);
+
+// Fills n 32-bit words with `value` (a kernel, so it is capturable).
+int FillU32(hd_context* ctx, uint32_t* p, uint32_t value, uint64_t n);
+int FillU64(hd_context* ctx, unsigned long long* p, unsigned long long value,
+            uint64_t n);
+
+// Repacks AoS points into SoA arrays of n_pad entries (n_pad >= n), visiting
+// source index perm(j mod n) for destination j (identity if !shuffle).
+// z may be null for 2D.
+int GatherSoA(hd_context* ctx, int n_dims, const float* aos, uint32_t n,
+              uint32_t n_pad, bool shuffle, const Feistel& perm,
+              float* x, float* y, float* z);
+
+// Tight MBR of an AoS point set into a device array lower[d], upper[d]
+// (single-pass reduction: wave, then block, then one atomic per block).
+int DeviceMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
+              float* d_mbr /* 2*n_dims floats, device */);
+
+// Reads the MBR back (synchronous) and applies the reference's
+// ToNonemptyMBR widening (src/geoms/mbr.h:288-296).
+int HostMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
+            float* mbr /* host, 2*n_dims */);
+
+}  // namespace hd
+
+#endif  // HD_PRIMITIVES_H

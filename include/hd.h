@@ -1,0 +1,241 @@
+/*
+ * hd.h — C ABI of the MI355X Hausdorff-distance library (libhd.so).
+ *
+ * Drop-in boundary for the reference's hot path:
+ *   void hd::RunHausdorffDistance(const RunConfig&)
+ *     reference src/run_hausdorff_distance.cuh:5, src/run_hausdorff_distance.cu:53-402
+ *   HausdorffDistance<T,N>::CalculateDistance(stream, d_a, d_b)
+ *     reference src/hausdorff_distance.h:15-42 (GPU overload :27-32)
+ * Plain C types only; no exception or abort crosses this boundary. Every
+ * entry point returns HD_OK (0) or a negative HD_E* code and leaves a
+ * message for hd_last_error() on the calling thread.
+ *
+ * Exactness contract. A directed result is
+ *   cmax2 = max over a in A of min over b in B of d2(a, b)
+ * with d2 the reference's FP32 formula (ax-bx)*(ax-bx)+(ay-by)*(ay-by)
+ * [+(az-bz)*(az-bz)], left to right and unfused (src/geoms/distance.h:17-24).
+ * Pairs with d2 >= FLT_MAX never count, and a point none of whose pairs
+ * counts is skipped, as the reference's FLT_MAX-initialised running min does
+ * (src/hd_impl/hausdorff_distance_early_break.h:69-82). Every variant returns
+ * the same cmax2 bits. The reference returns only sqrt(cmax2); this library
+ * also returns the arg-max (the smallest source index whose exact NN squared
+ * distance equals cmax2) and its NN (the smallest target index at that d2).
+ */
+#ifndef HD_H
+#define HD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HD_ABI_VERSION 1u
+
+/* Return codes. The reference aborts through glog CHECK / LOG(FATAL)
+ * (src/main.cpp:93,105,134; src/run_hausdorff_distance.cu:115-116,333) and
+ * throws hd::Exception on CUDA errors (src/utils/exception.h:104-160). */
+#define HD_OK 0
+#define HD_EINVAL (-1)        /* bad argument (n_dims, empty set, NaN, ...) */
+#define HD_ENOMEM (-2)        /* device or host allocation failed */
+#define HD_EDEVICE (-3)       /* HIP runtime or kernel launch error */
+#define HD_EUNSUPPORTED (-4)  /* valid request this build does not serve */
+#define HD_ETRUNC (-5)        /* stats JSON did not fit; result is valid */
+
+/* Variants. Reference enum Variant (src/run_config.h:7-16) and the CLI
+ * strings of src/main.cpp:76-94: "eb" -> EB, "rt" -> LBVH (the OptiX BVH
+ * query becomes a software LBVH), "hybrid" -> HYBRID. GRID is the uniform
+ * grid index queried on its own. */
+#define HD_VARIANT_EB 0
+#define HD_VARIANT_GRID 1
+#define HD_VARIANT_LBVH 2
+#define HD_VARIANT_HYBRID 3
+
+/* Execution (src/run_config.h:18). This library serves GPU execution only;
+ * HD_EXEC_CPU returns HD_EUNSUPPORTED. */
+#define HD_EXEC_CPU 0
+#define HD_EXEC_GPU 1
+
+/* hd_run_config.flags */
+#define HD_FLAG_SYMMETRIC 0x1u       /* H = max(h(A->B), h(B->A)) */
+#define HD_FLAG_CHECK 0x2u           /* re-run with the plain EB variant and
+                                        compare bits (cf. -check, :366-388) */
+#define HD_FLAG_LEARNED_ORDER 0x4u   /* visit A in descending MLP-predicted
+                                        NN distance */
+#define HD_FLAG_COUNT_EVALS 0x8u     /* count d2 evaluations */
+#define HD_FLAG_INPUT_ON_DEVICE 0x10u/* a, b are device pointers */
+#define HD_FLAG_NO_SHUFFLE 0x20u     /* visit A and B in input order */
+
+/* Reference RunConfig (src/run_config.h:20-61) restricted to the hot path,
+ * with the defaults of src/flags.cc:3-57 (hd_run_config_init). */
+typedef struct hd_run_config {
+  uint32_t abi_version;    /* HD_ABI_VERSION */
+  int32_t variant;         /* HD_VARIANT_*            (-variant, default eb) */
+  int32_t execution;       /* HD_EXEC_*               (-execution, gpu) */
+  int32_t n_dims;          /* 2 or 3                  (-n_dims) */
+  int32_t repeat;          /* >= 1                    (-repeat, 5) */
+  int32_t seed;            /* visit-order seed        (-seed, 0) */
+  int32_t parallelism;     /* host threads, <=0 = all (-parallelism, -1) */
+  uint32_t flags;          /* HD_FLAG_* */
+  float sample_rate;       /* hybrid sampling rate    (-sample_rate, 0.01) */
+  uint32_t n_points_cell;  /* grid points per cell    (-n_points_cell, 8) */
+  uint32_t max_hit;        /* index cells a query may visit before the point
+                              is handed to the EB kernel (-max_hit, 256) */
+  uint32_t eb_only_threshold; /* hybrid: EB only if sampled p80 iterations
+                              fall below this (-eb_only_threshold, 0) */
+  int32_t auto_tune;       /* 1: parameters from the autotuner (-auto_tune) */
+  int32_t reserved;
+  const void* model;       /* optional hd_mlp* for HD_FLAG_LEARNED_ORDER */
+} hd_run_config;
+
+/* One direction. */
+typedef struct hd_directed_result {
+  uint32_t cmax2_bits;  /* FP32 bits of cmax2 */
+  uint32_t argmax;      /* source index, UINT32_MAX if no point counted */
+  uint32_t nn;          /* target index, UINT32_MAX if no point counted */
+  int32_t variant_used; /* HD_VARIANT_* that produced it (hybrid picks) */
+  float hd;             /* sqrtf(cmax2) */
+  float reserved;
+  uint64_t evals;       /* d2 evaluations (HD_FLAG_COUNT_EVALS), else 0 */
+  double time_ms;       /* device time of this direction */
+} hd_directed_result;
+
+typedef struct hd_result {
+  float hd;             /* HDResult: directed h(A->B), or symmetric H */
+  uint32_t cmax2_bits;
+  uint32_t argmax;      /* in set `direction` names */
+  uint32_t nn;          /* in the other set */
+  int32_t direction;    /* 0: A->B attains it, 1: B->A (ties: 0) */
+  int32_t check_pass;   /* -1 not checked, 0 mismatch, 1 bit-identical */
+  uint32_t n_repeats;
+  uint32_t reserved;
+  hd_directed_result ab;
+  hd_directed_result ba; /* zeroed unless HD_FLAG_SYMMETRIC */
+  double avg_time_ms;    /* Running.AvgTime: mean ReportedTime */
+  double median_time_ms;
+  double min_time_ms;
+} hd_result;
+
+typedef struct hd_context hd_context;
+typedef struct hd_mlp hd_mlp;
+
+/* Thread-local message of the last failing call on this thread. */
+const char* hd_last_error(void);
+
+/* Fills the reference defaults (src/flags.cc:3-57): variant eb, execution
+ * gpu, repeat 5, seed 0, parallelism -1, sample_rate 0.01, n_points_cell 8,
+ * max_hit 256, eb_only_threshold 0. n_dims is left 0 (must be set). */
+void hd_run_config_init(hd_run_config* cfg);
+
+/* A context owns one HIP stream on `device`, cached device buffers and
+ * events; calls on one context are serialised by the caller. The reference
+ * creates one stream per run (src/utils/stream.h:13-27). */
+int hd_context_create(int device, hd_context** out);
+void hd_context_destroy(hd_context* ctx);
+/* The context's hipStream_t, as void*. */
+void* hd_context_stream(hd_context* ctx);
+
+/* Replaces hd::RunHausdorffDistance (src/run_hausdorff_distance.cu:53-402)
+ * for in-memory inputs: validates, uploads (unless INPUT_ON_DEVICE), runs
+ * `repeat` timed repetitions, optionally checks, and writes the reference's
+ * stats JSON keys (Input, Running{Repeats[{Repeat, Algorithm, Execution,
+ * ReportedTime, ...}], AvgTime}, HDResult, Check{HDAnswer, Diff, Pass}) into
+ * stats_json (NUL-terminated; HD_ETRUNC if stats_cap is too small).
+ * a, b: AoS FP32, n_dims floats per point. ctx may be NULL (a temporary
+ * context on device 0 is used). Inputs are never modified. */
+int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
+                              const float* a, uint64_t n_a, const float* b,
+                              uint64_t n_b, hd_result* out, char* stats_json,
+                              uint64_t stats_cap);
+
+/* One directed h(A->B) on device-resident AoS inputs; synchronous. The GPU
+ * overload CalculateDistance(stream, d_a, d_b) of the reference
+ * (src/hausdorff_distance.h:27-32), without its in-place shuffle
+ * (early_break.h:132-136): d_a and d_b are read only. `variant` is
+ * HD_VARIANT_*, `flags` takes LEARNED_ORDER, COUNT_EVALS, NO_SHUFFLE.
+ * `model` (may be NULL) is used with LEARNED_ORDER; NULL trains one. */
+int hd_directed_device(hd_context* ctx, int variant, int n_dims,
+                       const float* d_a, uint64_t n_a, const float* d_b,
+                       uint64_t n_b, uint32_t flags, int32_t seed,
+                       const hd_mlp* model, hd_directed_result* out);
+
+/* Exact nearest neighbour of every query in `ref` on the device: the
+ * per-point minimum the EB loop computes when it does not break. index:
+ * HD_VARIANT_EB (exhaustive scan), HD_VARIANT_GRID or HD_VARIANT_LBVH.
+ * d_out_d2[i] = FP32 min d2 (FLT_MAX if no pair counts), d_out_idx[i] = the
+ * smallest ref index attaining it (UINT32_MAX if none). */
+int hd_nn_device(hd_context* ctx, int index, int n_dims, const float* d_q,
+                 uint64_t n_q, const float* d_ref, uint64_t n_ref,
+                 float* d_out_d2, uint32_t* d_out_idx);
+
+/* Batched directed HD: pair p is A[offs_a[p] .. offs_a[p+1]) against
+ * B[offs_b[p] .. offs_b[p+1]), all device-resident (AoS, offsets in points,
+ * n_pairs + 1 entries each). One persistent-kernel launch captured in a HIP
+ * graph; tiles come from the autotuner. out: host array of n_pairs results
+ * (evals and time_ms are per batch, reported in out[0]). No counterpart in
+ * the reference, which runs one pair per process. */
+int hd_batch_directed_device(hd_context* ctx, int n_dims, const float* d_a,
+                             const uint64_t* d_offs_a, const float* d_b,
+                             const uint64_t* d_offs_b, uint32_t n_pairs,
+                             uint32_t flags, hd_directed_result* out);
+
+/* ---- learned visit order (no reference counterpart: the reference's
+ * selector uses gradient-boosted trees, src/models/tree_*.h) ----
+ * MLP n_dims -> 64 -> 64 -> 1, ReLU, BF16 weights with an FP32 master copy,
+ * FP32 accumulation. Input: coordinates normalised by the training set's
+ * MBR. Output: predicted log of the NN distance. */
+typedef struct hd_mlp_train_config {
+  uint32_t steps;        /* Adam steps (2000) */
+  uint32_t batch;        /* samples per step (4096) */
+  uint32_t n_samples;    /* subsample of A labelled exactly (1,000,000) */
+  uint32_t seed;         /* init + subsample seed (14) */
+  float lr;              /* Adam learning rate (3e-3) */
+  int32_t label_index;   /* HD_VARIANT_GRID / LBVH / EB for the exact labels */
+} hd_mlp_train_config;
+
+void hd_mlp_train_config_init(hd_mlp_train_config* cfg);
+
+/* Trains on exact NN distances of a subsample of d_a against d_b, on the
+ * GPU with this library's forward / backward / Adam kernels. final_loss
+ * (may be NULL) receives the mean loss of the last 100 steps. */
+int hd_mlp_train_device(hd_context* ctx, int n_dims, const float* d_a,
+                        uint64_t n_a, const float* d_b, uint64_t n_b,
+                        const hd_mlp_train_config* cfg, hd_mlp** out,
+                        float* final_loss);
+/* d_out[i] = predicted log NN distance of d_pts[i] (FP32). */
+int hd_mlp_predict_device(hd_context* ctx, const hd_mlp* model,
+                          const float* d_pts, uint64_t n, float* d_out);
+void hd_mlp_destroy(hd_mlp* model);
+/* Parameter count and FP32 master parameters (layout W1[64][d], b1[64],
+ * W2[64][64], b2[64], w3[64], b3[1]); also the input normalisation
+ * (lower[d], inv_extent[d]) and label shift/scale. */
+int hd_mlp_n_params(int n_dims);
+int hd_mlp_get_params(const hd_mlp* model, float* params, float* norm_lower,
+                      float* norm_inv_extent);
+/* One forward + backward pass on the GPU for a host batch with host FP32
+ * parameters: writes the mean-squared-error loss and the FP32 gradient.
+ * Weights are rounded to BF16 first, as in training. Test hook for the
+ * gradient check against an FP64 CPU evaluation. */
+int hd_mlp_loss_grad_device(hd_context* ctx, int n_dims, const float* params,
+                            const float* x, const float* y, uint32_t batch,
+                            float* loss, float* grad);
+
+/* Per-stage device timing. With profiling on, each pipeline stage (kernel or
+ * library call) is bracketed by HIP events on the context stream;
+ * hd_context_stage_times writes {"stage": {"ms": total, "calls": n,
+ * "evals": d2 evaluations, "points": points processed}, ...} accumulated
+ * since the last reset, then resets if `reset`. No reference counterpart
+ * (the reference times whole calls with a host Stopwatch,
+ * src/utils/stopwatch.h). */
+int hd_context_set_profiling(hd_context* ctx, int enable);
+int hd_context_stage_times(hd_context* ctx, char* json, uint64_t cap,
+                           int reset);
+
+/* Device / build information. */
+int hd_device_count(void);
+/* Writes a short JSON with device name, CU count, clock, build flags. */
+int hd_device_info(int device, char* buf, uint64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HD_H */

@@ -1,0 +1,110 @@
+"""GPU parity of the early-break variant through the C ABI: cmax2 bits,
+arg-max and NN index must equal the CPU oracle's on the same inputs."""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import hausdorffdistance_amd as hd
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def run(a, b, variant=hd.Variant.EB, symmetric=True, repeat=1, ctx=None, **kw):
+    cfg = hd.RunConfig(variant=variant, n_dims=a.shape[1], repeat=repeat,
+                       symmetric=symmetric, **kw)
+    return hd.run_hausdorff_distance(cfg, a, b, ctx=ctx)
+
+
+def triple(r):
+    return {"cmax2_bits": r.cmax2_bits, "argmax": r.argmax, "nn": r.nn}
+
+
+def test_golden_fixtures(ctx):
+    cases = np.load(GOLDEN / "cases.npz")
+    expected = json.loads((GOLDEN / "expected.json").read_text())
+    for name, exp in expected.items():
+        a, b = cases[f"{name}__a"], cases[f"{name}__b"]
+        print(f"[golden] {name} {a.shape} {b.shape}", flush=True)
+        r = run(a, b, ctx=ctx)
+        assert triple(r.ab) == exp["ab"], name
+        assert triple(r.ba) == exp["ba"], name
+
+
+@pytest.mark.parametrize("n_dims", [2, 3])
+@pytest.mark.parametrize("na,nb", [(1, 1), (1, 777), (777, 1), (63, 65), (64, 64),
+                                   (1000, 3000), (4097, 129)])
+def test_small_sizes_match_brute_force(ctx, n_dims, na, nb):
+    a = hd.gen.uniform(100 + na, na, n_dims)
+    b = hd.gen.uniform(200 + nb, nb, n_dims)
+    r = run(a, b, ctx=ctx)
+    assert triple(r.ab) == triple(oracle.brute_directed(a, b))
+    assert triple(r.ba) == triple(oracle.brute_directed(b, a))
+
+
+def test_uniform_100k_2d_matches_cpu_early_break(ctx):
+    a, b = hd.gen.uniform_pair(100_000, 2)
+    r = run(a, b, ctx=ctx, repeat=2)
+    o_ab, o_ba = oracle.eb_symmetric(a, b)
+    assert triple(r.ab) == triple(o_ab)
+    assert triple(r.ba) == triple(o_ba)
+    assert r.hd == max(o_ab.hd, o_ba.hd)
+    assert r.stats["Running"]["Repeats"][0]["Algorithm"] == "Early Break"
+    assert r.stats["HDResult"] == pytest.approx(r.hd)
+
+
+def test_gmm_200k_2d_both_directions(ctx):
+    a, b = hd.gen.gmm_pair(200_000)
+    r = run(a, b, ctx=ctx)
+    o_ab, o_ba = oracle.eb_symmetric(a, b)
+    assert triple(r.ab) == triple(o_ab)
+    assert triple(r.ba) == triple(o_ba)
+    assert r.direction == (1 if o_ba.cmax2_bits > o_ab.cmax2_bits else 0)
+
+
+def test_sphere_3d_matches(ctx):
+    a = hd.gen.sphere(7, 60_000, 0.005, 0.0)
+    b = hd.gen.sphere(8, 45_000, 0.005, 2.0)
+    r = run(a, b, ctx=ctx)
+    o_ab, o_ba = oracle.eb_symmetric(a, b)
+    assert triple(r.ab) == triple(o_ab)
+    assert triple(r.ba) == triple(o_ba)
+
+
+def test_order_does_not_change_bits(ctx):
+    a, b = hd.gen.gmm_pair(50_000)
+    ref = oracle.brute_directed(a, b) if False else oracle.eb_directed(a, b)
+    for seed in (0, 1, 12345):
+        for shuffle in (True, False):
+            r = run(a, b, ctx=ctx, symmetric=False, seed=seed, shuffle=shuffle)
+            assert triple(r.ab) == triple(ref), (seed, shuffle)
+
+
+def test_device_inputs_and_class_interface(ctx):
+    torch = pytest.importorskip("torch")
+    a = hd.gen.uniform(5, 20_000, 3)
+    b = hd.gen.uniform(6, 15_000, 3)
+    da = torch.from_numpy(a).cuda()
+    db = torch.from_numpy(b).cuda()
+    eb = hd.HausdorffDistanceEarlyBreak(ctx=ctx)
+    dist = eb.calculate_distance(da, db)
+    o = oracle.eb_directed(a, b)
+    assert eb.last.cmax2_bits == o.cmax2_bits and eb.last.argmax == o.argmax
+    assert dist == pytest.approx(o.hd)
+    st = eb.get_stats()
+    assert st["Algorithm"] == "Early Break" and st["Execution"] == "GPU"
+    assert st["ReportedTime"] > 0
+    # inputs are not modified
+    assert np.array_equal(da.cpu().numpy(), a)
+
+
+def test_check_flag_runs_plain_eb(ctx):
+    a, b = hd.gen.uniform_pair(20_000, 2)
+    r = run(a, b, ctx=ctx, check=True)
+    assert r.check_pass == 1
+    assert r.stats["Check"]["Pass"] is True

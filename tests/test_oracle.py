@@ -1,0 +1,226 @@
+"""Pins the CPU oracle (oracle/hd_oracle.c) before anything is checked
+against it: IEEE known answers for the squared distance, exhaustive
+max-min on tiny sets against an independent pure-Python evaluation, closed
+forms, invariants and order independence."""
+from __future__ import annotations
+
+import itertools
+import json
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+import hausdorffdistance_amd.gen as gen
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def f32(x) -> np.float32:
+    """Correct rounding of an exact rational to FP32 (round half even)."""
+    x = Fraction(x)
+    if x == 0:
+        return np.float32(0.0)
+    sign = -1 if x < 0 else 1
+    x = abs(x)
+    e = x.numerator.bit_length() - x.denominator.bit_length()
+    if Fraction(2) ** e > x:
+        e -= 1
+    e = max(e, -126)  # subnormal range keeps the exponent fixed
+    ulp = Fraction(2) ** (e - 23)
+    q, r = divmod(x, ulp)
+    q = int(q)
+    if r > ulp / 2 or (r == ulp / 2 and q % 2 == 1):
+        q += 1
+    return np.float32(sign * float(q * ulp))
+
+
+def d2_unfused(p, q) -> np.float32:
+    """The reference's expression evaluated with one rounding per operation,
+    via exact rationals (independent of any FP hardware)."""
+    acc = None
+    for pi, qi in zip(p, q):
+        diff = f32(Fraction(float(np.float32(pi))) - Fraction(float(np.float32(qi))))
+        sq = f32(Fraction(float(diff)) ** 2)
+        acc = sq if acc is None else f32(Fraction(float(acc)) + Fraction(float(sq)))
+    return acc
+
+
+def d2_fused(p, q) -> np.float32:
+    """What an FMA-contracted compile would compute for 2D: fma(dx, dx, dy*dy)."""
+    dx = f32(Fraction(float(np.float32(p[0]))) - Fraction(float(np.float32(q[0]))))
+    dy = f32(Fraction(float(np.float32(p[1]))) - Fraction(float(np.float32(q[1]))))
+    return f32(Fraction(float(dx)) ** 2 + Fraction(float(f32(Fraction(float(dy)) ** 2))))
+
+
+def test_d2_matches_unfused_ieee_and_not_fma():
+    rng = np.random.default_rng(0)
+    n_fused_differs = 0
+    for _ in range(400):
+        p = rng.random(2).astype(np.float32)
+        q = rng.random(2).astype(np.float32)
+        want = d2_unfused(p, q)
+        got = oracle.d2(p, q)
+        assert got.view(np.uint32) == want.view(np.uint32), (p, q, got, want)
+        if d2_fused(p, q).view(np.uint32) != want.view(np.uint32):
+            n_fused_differs += 1
+    # The sample contains inputs where contraction would change the bits, so
+    # a contracted build of the oracle fails this test.
+    assert n_fused_differs > 10
+
+
+def test_d2_3d_and_denormal():
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        p = rng.random(3).astype(np.float32)
+        q = rng.random(3).astype(np.float32)
+        assert oracle.d2(p, q).view(np.uint32) == d2_unfused(p, q).view(np.uint32)
+    tiny = oracle.d2([1e-20, 0], [0, 0])
+    assert 0 < tiny < np.finfo(np.float32).tiny  # subnormal kept, not flushed
+
+
+def python_max_min(a: np.ndarray, b: np.ndarray):
+    """Exhaustive max-min with the documented tie rules, pure Python."""
+    best = None  # (d2 bits, -index)
+    mins = []
+    for i, pa in enumerate(a):
+        m = None
+        for pb in b:
+            d = d2_unfused(pa, pb)
+            if d < np.float32(np.finfo(np.float32).max) and (m is None or d < m):
+                m = d
+        mins.append(m)
+        if m is not None:
+            key = (int(m.view(np.uint32)), -i)
+            if best is None or key > best:
+                best = key
+    if best is None:
+        return 0, 0xFFFFFFFF, 0xFFFFFFFF
+    argmax = -best[1]
+    nn = next(j for j, pb in enumerate(b)
+              if int(d2_unfused(a[argmax], pb).view(np.uint32)) == best[0])
+    return best[0], argmax, nn
+
+
+@pytest.mark.parametrize("n_dims,na,nb,seed", [(2, 9, 7, 0), (3, 6, 11, 1), (2, 1, 5, 2),
+                                                 (2, 5, 1, 3)])
+def test_brute_and_eb_match_pure_python(n_dims, na, nb, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.random((na, n_dims)).astype(np.float32)
+    b = rng.random((nb, n_dims)).astype(np.float32)
+    want = python_max_min(a, b)
+    r = oracle.brute_directed(a, b)
+    assert (r.cmax2_bits, r.argmax, r.nn) == want
+    for threads, s in itertools.product([1, 3], [0, 5]):
+        e = oracle.eb_directed(a, b, n_threads=threads, seed=s)
+        assert (e.cmax2_bits, e.argmax, e.nn) == want
+
+
+def test_ties_pick_smallest_index():
+    # Points 2 and 4 tie at the maximum; 4 appears first after shuffling for
+    # many seeds, but the smallest index must win.
+    a = np.array([[0, 0], [1, 0], [5, 0], [2, 0], [5, 8]], np.float32)
+    b = np.array([[0, 0], [1, 0], [2, 0], [5, 5]], np.float32)
+    want = python_max_min(a, b)
+    assert want[1] == 2
+    for s in range(20):
+        e = oracle.eb_directed(a, b, n_threads=2, seed=s)
+        assert (e.cmax2_bits, e.argmax, e.nn) == want
+
+
+def test_order_and_thread_independence():
+    a = gen.uniform(21, 3000, 2)
+    b = gen.uniform(22, 2500, 2)
+    ref = oracle.brute_directed(a, b)
+    for threads in (1, 2, 8):
+        for seed in (0, 1, 99):
+            e = oracle.eb_directed(a, b, n_threads=threads, seed=seed)
+            assert (e.cmax2_bits, e.argmax, e.nn) == (ref.cmax2_bits, ref.argmax, ref.nn)
+    e = oracle.eb_directed(a, b, n_threads=1, shuffle=False)
+    assert e.cmax2_bits == ref.cmax2_bits
+    assert e.compared_pairs < a.shape[0] * b.shape[0]  # early break does break
+
+
+def test_closed_forms():
+    a = gen.uniform(31, 400, 2)
+    # A = B: every min is 0.
+    r = oracle.eb_directed(a, a.copy())
+    assert (r.cmax2_bits, r.argmax, r.nn) == (0, 0, 0)
+    # Single A point: h is its NN distance.
+    p = np.array([[0.3, 0.6]], np.float32)
+    r = oracle.eb_directed(p, a)
+    d = np.array([oracle.d2(p[0], q) for q in a], np.float32)
+    assert r.cmax2_bits == int(d.min().view(np.uint32)) and r.nn == int(np.argmin(d))
+    # Power-of-two lattice shifted by an exactly representable offset.
+    ys, xs = np.meshgrid(np.arange(32), np.arange(32), indexing="ij")
+    lat = (np.stack([xs.ravel(), ys.ravel()], 1) / 32.0).astype(np.float32)
+    sh = lat + np.float32(1 / 256)
+    sh[:, 1] = lat[:, 1]
+    r = oracle.eb_directed(lat, sh)
+    assert r.cmax2_bits == int(np.float32(2.0 ** -16).view(np.uint32))
+    assert r.argmax == 0 and r.nn == 0
+
+
+def test_invariants_symmetric_and_subset():
+    a = gen.uniform(41, 1500, 3)
+    b = gen.uniform(42, 1200, 3)
+    c = gen.uniform(43, 800, 3)
+    ab, ba = oracle.eb_symmetric(a, b)
+    sym = max(ab.cmax2_bits, ba.cmax2_bits)
+    assert sym >= ab.cmax2_bits and sym >= ba.cmax2_bits
+    # Adding points to B can only lower h(A -> B).
+    r_bc = oracle.eb_directed(a, np.concatenate([b, c]))
+    assert r_bc.cmax2_bits <= ab.cmax2_bits
+
+
+def test_hd_bounds_bracket_the_distance():
+    for seed in range(5):
+        a = gen.uniform(50 + seed, 700, 2) * np.float32(0.5)
+        b = gen.uniform(60 + seed, 600, 2) * np.float32(0.5) + np.float32(0.3)
+        r = oracle.eb_directed(a, b)
+        lb2, ub2 = oracle.hd_bounds2(oracle.mbr(a), oracle.mbr(b), 2)
+        assert lb2 <= r.cmax2 * (1 + 1e-6) and r.cmax2 <= ub2 * (1 + 1e-6)
+
+
+def test_mbr_matches_numpy():
+    p = gen.uniform(70, 1000, 3) - np.float32(0.5)
+    m = oracle.mbr(p)
+    np.testing.assert_array_equal(m[:3], p.min(0))
+    np.testing.assert_array_equal(m[3:], p.max(0))
+
+
+def test_golden_fixtures_against_oracle():
+    cases = np.load(GOLDEN / "cases.npz")
+    expected = json.loads((GOLDEN / "expected.json").read_text())
+    assert len(expected) >= 10
+    for name, exp in expected.items():
+        a, b = cases[f"{name}__a"], cases[f"{name}__b"]
+        for direction, (x, y) in (("ab", (a, b)), ("ba", (b, a))):
+            r = oracle.eb_directed(x, y, n_threads=3, seed=1)
+            got = {"cmax2_bits": r.cmax2_bits, "argmax": r.argmax, "nn": r.nn}
+            assert got == exp[direction], (name, direction)
+
+
+def test_mlp_gradient_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    d_in, hidden = 2, 8
+    n = hidden * d_in + hidden + hidden * hidden + hidden + hidden + 1
+    params = (rng.standard_normal(n) * 0.5).astype(np.float32)
+    x = rng.random((16, d_in)).astype(np.float32)
+    y = rng.random(16).astype(np.float32)
+    loss, g = oracle.mlp_loss_grad(params, d_in, hidden, x, y)
+    for k in rng.choice(n, 25, replace=False):
+        h = 1e-3
+        pp = params.astype(np.float64).copy()
+        pm = params.astype(np.float64).copy()
+        pp[k] += h
+        pm[k] -= h
+        lp, _ = oracle.mlp_loss_grad(pp.astype(np.float32), d_in, hidden, x, y)
+        lm, _ = oracle.mlp_loss_grad(pm.astype(np.float32), d_in, hidden, x, y)
+        # central difference on FP32-rounded parameters
+        dp = float(np.float32(pp[k])) - float(np.float32(pm[k]))
+        fd = (lp - lm) / dp
+        assert abs(fd - g[k]) <= 2e-3 * max(1.0, abs(g[k])), (k, fd, g[k])
