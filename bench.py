@@ -127,6 +127,7 @@ def main() -> None:
     r0 = res_plain[-1]
     log(f"plain: {ms_plain:.3f} ms/step, cmax2 bits {r0.cmax2_bits} argmax {r0.argmax} "
         f"evals {r0.evals}")
+    log(f"plain stages: {json.dumps(st_plain)}")
     ms_learned = None
     st_learned = None
     r1 = None

@@ -168,7 +168,12 @@ class HDError(RuntimeError):
 
 
 def load() -> C.CDLL:
-    """Loads libhd.so; raises if it has not been built."""
+    """Loads libhd.so; raises if it has not been built.
+
+    libhd.so binds to whichever HIP runtime with soname libamdhip64.so.7 is
+    already in the process (torch ships its own under that soname); to pass
+    torch tensors, import torch before the first call here so both share one
+    runtime."""
     global _lib
     if _lib is not None:
         return _lib

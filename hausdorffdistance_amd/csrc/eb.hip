@@ -62,7 +62,9 @@ HD_DEV uint32_t OrderAt(int mode, const uint32_t* order, const Feistel& perm,
 
 struct SlotArgs {
   const float* a;         // AoS A
-  const uint32_t* order;  // visit order, or null for the Feistel order
+  const uint32_t* order;  // visit order (kOrderArray)
+  const float* list_min;  // optional partial minima per position: a position
+                          // whose value is already below cmax2 is skipped
   Feistel perm_a;
   int order_mode;         // kOrder*
   uint32_t pos_end;       // order positions [cursor start, pos_end)
@@ -86,114 +88,184 @@ HD_DEV void LoadPoint(const float* __restrict__ a, uint32_t idx, float* p) {
   for (int d = 0; d < ND; ++d) p[d] = Uniform(src[d]);
 }
 
+HD_DEV float ReadLaneF(float v, uint32_t k) {
+  return __uint_as_float(
+      __builtin_amdgcn_readlane(__float_as_uint(v), (int)k));
+}
+HD_DEV uint32_t ReadLaneU(uint32_t v, uint32_t k) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k);
+}
+
+// B step j (64 SoA points) into registers.
+template <int ND>
+HD_DEV void LoadStep(const float* __restrict__ bx, const float* __restrict__ by,
+                     const float* __restrict__ bz, uint32_t step,
+                     uint32_t lane, float* b) {
+  const uint32_t j = (step << 6) + lane;
+  b[0] = bx[j];
+  b[1] = by[j];
+  if constexpr (ND == 3) b[2] = bz[j];
+}
+
+template <int ND, int P>
+HD_DEV void EvalStep(const float (&pa)[P][ND], uint32_t live_mask,
+                     const float* b, float (&lmin)[P]) {
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    if (live_mask & (1u << p)) {
+      float d;
+      if constexpr (ND == 2)
+        d = D2(pa[p][0], pa[p][1], b[0], b[1]);
+      else
+        d = D2(pa[p][0], pa[p][1], pa[p][2], b[0], b[1], b[2]);
+      lmin[p] = fminf(lmin[p], d);
+    }
+  }
+}
+
+HD_DEV uint32_t NextStep(uint32_t s, uint32_t n) { return s + 1 == n ? 0 : s + 1; }
+
 // Stage 2. One wave = P slots; block = 4 independent waves.
+// Refill goes through a per-wave reservoir: 64 order positions claimed with
+// one atomic, each lane loading one candidate point in parallel; slots then
+// take points from the reservoir with readlane.
 template <int ND, int P>
 __global__ __launch_bounds__(kBlock) void EbSlotKernel(SlotArgs args) {
   const uint32_t lane = LaneId();
   const uint32_t wave_id = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-  uint32_t step = (uint32_t)(((uint64_t)wave_id * 7919u) % args.n_steps);
+  const uint32_t n_steps = args.n_steps;
+  uint32_t step = (uint32_t)(((uint64_t)wave_id * 7919u) % n_steps);
 
   float pa[P][ND];
   uint32_t aidx[P];
-  uint32_t done_steps[P];
-  bool live[P];
+  uint32_t start[P];
   float lmin[P];
-  uint32_t res_base = 0, res_left = 0;
-  bool drained = false;
-  unsigned long long evals = 0;
-
+  uint32_t live = 0;  // bit p: slot p holds a point
 #pragma unroll
   for (int p = 0; p < P; ++p) {
-    live[p] = false;
-    lmin[p] = kFltMax;
-    done_steps[p] = 0;
     aidx[p] = 0;
+    start[p] = 0;
+    lmin[p] = kFltMax;
 #pragma unroll
     for (int d = 0; d < ND; ++d) pa[p][d] = 0.f;
   }
+  // Reservoir (one candidate per lane) and its valid-lane mask.
+  float rp[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) rp[d] = 0.f;
+  uint32_t ridx = 0;
+  unsigned long long rmask = 0;
+  bool drained = false;
+  uint32_t gstep = 0;
+  unsigned long long evals = 0;
+  // Escalations staged in lanes (lane k holds the k-th pending one) and
+  // appended 64 at a time, so the global counter sees one atomic per 64.
+  uint32_t stg_idx = 0;
+  float stg_min = 0.f;
+  uint32_t n_stg = 0;
+  auto flush = [&]() {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(args.esc_count, n_stg);
+    base = Uniform(base);
+    if (lane < n_stg) {
+      args.esc_idx[base + lane] = stg_idx;
+      args.esc_min[base + lane] = stg_min;
+    }
+    n_stg = 0;
+  };
+
+  float cmax2 = LoadCmax2(args.key);
+  float bcur[2][ND];
+  LoadStep<ND>(args.bx, args.by, args.bz, step, lane, bcur[0]);
+  LoadStep<ND>(args.bx, args.by, args.bz, NextStep(step, n_steps), lane,
+               bcur[1]);
 
   while (true) {
-    // Refill empty slots from the wave's reservoir of order positions.
+    // Refill.
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-      if (!live[p] && !drained) {
-        if (res_left == 0) {
+      if (!(live & (1u << p))) {
+        while (rmask == 0ull && !drained) {
           uint32_t base = 0;
           if (lane == 0) base = atomicAdd(args.cursor, (unsigned)kWaveSize);
           base = Uniform(base);
           if (base >= args.pos_end) {
             drained = true;
-          } else {
-            res_base = base;
-            res_left = min((uint32_t)kWaveSize, args.pos_end - base);
+            break;
           }
+          const uint32_t pos = base + lane;
+          bool valid = pos < args.pos_end;
+          if (valid && args.list_min != nullptr)
+            valid = !(args.list_min[pos] < cmax2);
+          if (valid) {
+            ridx = OrderAt(args.order_mode, args.order, args.perm_a, pos);
+            const float* src = args.a + (uint64_t)ridx * ND;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) rp[d] = src[d];
+          }
+          rmask = __ballot(valid);
         }
-        if (!drained) {
-          const uint32_t pos = res_base;
-          res_base++;
-          res_left--;
-          const uint32_t idx = Uniform(
-              OrderAt(args.order_mode, args.order, args.perm_a, pos));
-          aidx[p] = idx;
-          LoadPoint<ND>(args.a, idx, pa[p]);
+        if (rmask != 0ull) {
+          const uint32_t k = (uint32_t)__builtin_ctzll(rmask);
+          rmask &= rmask - 1ull;
+#pragma unroll
+          for (int d = 0; d < ND; ++d) pa[p][d] = ReadLaneF(rp[d], k);
+          aidx[p] = ReadLaneU(ridx, k);
           lmin[p] = kFltMax;
-          done_steps[p] = 0;
-          live[p] = true;
+          start[p] = gstep;
+          live |= 1u << p;
         }
       }
     }
-    int n_live = 0;
-#pragma unroll
-    for (int p = 0; p < P; ++p) n_live += live[p] ? 1 : 0;
-    if (n_live == 0) break;
+    if (live == 0) break;
+    const uint32_t n_live = __builtin_popcount(live);
 
-    // Two steps of 64 B points, then one break check.
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const uint32_t j = (step << 6) + lane;
-      const float bxv = args.bx[j];
-      const float byv = args.by[j];
-      const float bzv = ND == 3 ? args.bz[j] : 0.f;
-#pragma unroll
-      for (int p = 0; p < P; ++p) {
-        if (live[p]) {
-          float d;
-          if constexpr (ND == 2)
-            d = D2(pa[p][0], pa[p][1], bxv, byv);
-          else
-            d = D2(pa[p][0], pa[p][1], pa[p][2], bxv, byv, bzv);
-          lmin[p] = fminf(lmin[p], d);
-        }
-      }
-      step = (step + 1 == args.n_steps) ? 0 : step + 1;
-    }
+    // Prefetch the next two steps and cmax2, then evaluate the current two.
+    const uint32_t s2 = NextStep(NextStep(step, n_steps), n_steps);
+    const uint32_t s3 = NextStep(s2, n_steps);
+    float bnext[2][ND];
+    LoadStep<ND>(args.bx, args.by, args.bz, s2, lane, bnext[0]);
+    LoadStep<ND>(args.bx, args.by, args.bz, s3, lane, bnext[1]);
+    const uint32_t* cm_ptr = reinterpret_cast<const uint32_t*>(args.key) + 1;
+    const uint32_t cm_next =
+        __hip_atomic_load(cm_ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    EvalStep<ND, P>(pa, live, bcur[0], lmin);
+    EvalStep<ND, P>(pa, live, bcur[1], lmin);
+    step = s2;
+    gstep += 2;
     evals += (unsigned long long)n_live * 128ull;
 
-    const float cmax2 = LoadCmax2(args.key);
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-      if (live[p]) {
-        done_steps[p] += 2;
+      if (live & (1u << p)) {
+        const uint32_t done = gstep - start[p];
         if (WaveAny(lmin[p] < cmax2)) {
-          live[p] = false;  // cannot raise the maximum
-        } else if (done_steps[p] >= args.n_steps) {
+          live &= ~(1u << p);  // cannot raise the maximum
+        } else if (done >= n_steps) {
           // Whole of B scanned: exact minimum.
           const float m = WaveMin(lmin[p]);
           if (lane == 0 && m != kFltMax)
             atomicMax(args.key, PackKey(m, aidx[p]));
-          live[p] = false;
-        } else if (done_steps[p] >= args.budget_steps) {
+          live &= ~(1u << p);
+        } else if (done >= args.budget_steps) {
           const float m = WaveMin(lmin[p]);
-          if (lane == 0) {
-            const unsigned slot = atomicAdd(args.esc_count, 1u);
-            args.esc_idx[slot] = aidx[p];
-            args.esc_min[slot] = m;
+          if (lane == n_stg) {
+            stg_idx = aidx[p];
+            stg_min = m;
           }
-          live[p] = false;
+          if (++n_stg == (uint32_t)kWaveSize) flush();
+          live &= ~(1u << p);
         }
       }
     }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      bcur[0][d] = bnext[0][d];
+      bcur[1][d] = bnext[1][d];
+    }
+    cmax2 = __uint_as_float(Uniform(cm_next));
   }
+  if (n_stg > 0) flush();
   if (lane == 0) atomicAdd(args.evals, evals);
 }
 
@@ -220,19 +292,16 @@ struct CoopArgs {
 HD_DEV void MarkBroken(uint32_t* flag) {
   __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-HD_DEV uint32_t LoadFlag(const uint32_t* flag) {
-  return Uniform(
-      __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
 
-// Stages 1 and 3: work item = (group of P list entries, B segment); items of
-// one group are consecutive, so a group is spread over many waves at once.
+// Stages 1, 3 and the tail: work item = (group of P list entries, B
+// segment); items of one group are consecutive, so a group is spread over
+// many waves at once. Entry data is loaded lane-parallel (lane p < P loads
+// entry p) and handed to the uniform slots with readlane.
 template <int ND, int P>
 __global__ __launch_bounds__(kBlock) void EbCoopKernel(CoopArgs args) {
   const uint32_t lane = LaneId();
+  const uint32_t n_steps = args.n_steps;
   float pa[P][ND];
-  uint32_t aidx[P];
-  bool live[P];
   float lmin[P];
   unsigned long long evals = 0;
 
@@ -247,85 +316,86 @@ __global__ __launch_bounds__(kBlock) void EbCoopKernel(CoopArgs args) {
     const uint32_t s = (uint32_t)(t % args.n_segs);
 
     float cmax2 = LoadCmax2(args.key);
-    int n_live = 0;
+    // Lane-parallel entry load.
+    const uint32_t e_lane = g * P + lane;
+    bool ok = lane < (uint32_t)P && e_lane < args.n_list;
+    float ep[ND];
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-      const uint32_t e = g * P + p;
-      live[p] = false;
-      aidx[p] = 0;
-      lmin[p] = kFltMax;
+    for (int d = 0; d < ND; ++d) ep[d] = 0.f;
+    if (ok) {
+      const uint32_t flag = __hip_atomic_load(
+          &args.broken[e_lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (flag != 0u) {
+        ok = false;
+      } else if (args.list_min != nullptr && args.list_min[e_lane] < cmax2) {
+        MarkBroken(&args.broken[e_lane]);
+        ok = false;
+      } else {
+        const float* src = args.a + (uint64_t)args.list[e_lane] * ND;
 #pragma unroll
-      for (int d = 0; d < ND; ++d) pa[p][d] = 0.f;
-      if (e < args.n_list) {
-        if (LoadFlag(&args.broken[e]) == 0u) {
-          if (args.list_min != nullptr && Uniform(args.list_min[e]) < cmax2) {
-            if (lane == 0) MarkBroken(&args.broken[e]);
-          } else {
-            aidx[p] = Uniform(args.list[e]);
-            LoadPoint<ND>(args.a, aidx[p], pa[p]);
-            live[p] = true;
-            n_live++;
-          }
-        }
+        for (int d = 0; d < ND; ++d) ep[d] = src[d];
       }
     }
-    if (n_live == 0) {
+    uint32_t live = (uint32_t)__ballot(ok);
+    if (live == 0) {
       // Every entry of the group is decided: skip its remaining segments.
       if (lane == 0)
         atomicMax(args.claim, (unsigned long long)(g + 1) * args.n_segs);
       continue;
     }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      lmin[p] = kFltMax;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) pa[p][d] = ReadLaneF(ep[d], p);
+    }
 
     const uint32_t st_begin = s * args.seg_steps;
-    const uint32_t st_end = min(st_begin + args.seg_steps, args.n_steps);
+    const uint32_t st_end = min(st_begin + args.seg_steps, n_steps);
     uint32_t since_flags = 0;
-    for (uint32_t st = st_begin; st < st_end; st += 2) {
-      const bool two = st + 1 < st_end;
+    float bcur[ND];
+    LoadStep<ND>(args.bx, args.by, args.bz, st_begin, lane, bcur);
+    const uint32_t* cm_ptr = reinterpret_cast<const uint32_t*>(args.key) + 1;
+    for (uint32_t st = st_begin; st < st_end; ++st) {
+      // Prefetch the next step (clamped inside the segment) and cmax2.
+      float bnext[ND];
+      LoadStep<ND>(args.bx, args.by, args.bz, st + 1 < st_end ? st + 1 : st,
+                   lane, bnext);
+      const uint32_t cm_next = __hip_atomic_load(
+          cm_ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      EvalStep<ND, P>(pa, live, bcur, lmin);
+      evals += (unsigned long long)__builtin_popcount(live) * 64ull;
+      if (((st - st_begin) & 1u) == 1u || st + 1 == st_end) {
+        // Break checks every two steps.
+        const bool refresh = (++since_flags & 7u) == 0u;
+        bool flagged = false;
+        if (refresh && lane < (uint32_t)P && (live & (1u << lane)))
+          flagged = __hip_atomic_load(&args.broken[g * P + lane],
+                                      __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) != 0u;
+        const uint32_t flagged_mask = (uint32_t)__ballot(flagged);
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        if (u == 0 || two) {
-          const uint32_t j = ((st + u) << 6) + lane;
-          const float bxv = args.bx[j];
-          const float byv = args.by[j];
-          const float bzv = ND == 3 ? args.bz[j] : 0.f;
-#pragma unroll
-          for (int p = 0; p < P; ++p) {
-            if (live[p]) {
-              float d;
-              if constexpr (ND == 2)
-                d = D2(pa[p][0], pa[p][1], bxv, byv);
-              else
-                d = D2(pa[p][0], pa[p][1], pa[p][2], bxv, byv, bzv);
-              lmin[p] = fminf(lmin[p], d);
+        for (int p = 0; p < P; ++p) {
+          if (live & (1u << p)) {
+            if (WaveAny(lmin[p] < cmax2)) {
+              if (lane == 0) MarkBroken(&args.broken[g * P + p]);
+              live &= ~(1u << p);
+            } else if (flagged_mask & (1u << p)) {
+              live &= ~(1u << p);  // another segment broke it
             }
           }
         }
+        if (live == 0) break;
       }
-      evals += (unsigned long long)n_live * (two ? 128ull : 64ull);
-      cmax2 = LoadCmax2(args.key);
-      const bool refresh = (++since_flags & 7u) == 0u;
-      n_live = 0;
 #pragma unroll
-      for (int p = 0; p < P; ++p) {
-        if (live[p]) {
-          const uint32_t e = g * P + p;
-          if (WaveAny(lmin[p] < cmax2)) {
-            if (lane == 0) MarkBroken(&args.broken[e]);
-            live[p] = false;
-          } else if (refresh && LoadFlag(&args.broken[e]) != 0u) {
-            live[p] = false;  // another segment broke it
-          } else {
-            n_live++;
-          }
-        }
-      }
-      if (n_live == 0) break;
+      for (int d = 0; d < ND; ++d) bcur[d] = bnext[d];
+      cmax2 = __uint_as_float(Uniform(cm_next));
     }
 
     // Publish segment minima, then count the segment as done.
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-      if (live[p]) {
+      if (live & (1u << p)) {
         const float m = WaveMin(lmin[p]);
         if (lane == 0) atomicMin(&args.pmin[g * P + p], __float_as_uint(m));
       }
@@ -335,15 +405,13 @@ __global__ __launch_bounds__(kBlock) void EbCoopKernel(CoopArgs args) {
       prev = __hip_atomic_fetch_add(&args.gdone[g], 1u, __ATOMIC_ACQ_REL,
                                     __HIP_MEMORY_SCOPE_AGENT);
     prev = Uniform(prev);
-    if (prev + 1 == args.n_segs && lane == 0) {
+    if (prev + 1 == args.n_segs && lane < (uint32_t)P) {
       // Last segment of the group: every segment has published; entries
       // never marked broken were scanned over all of B.
-      for (int p = 0; p < P; ++p) {
-        const uint32_t e = g * P + p;
-        if (e >= args.n_list) break;
-        if (__hip_atomic_load(&args.broken[e], __ATOMIC_ACQUIRE,
-                              __HIP_MEMORY_SCOPE_AGENT) != 0u)
-          continue;
+      const uint32_t e = g * P + lane;
+      if (e < args.n_list &&
+          __hip_atomic_load(&args.broken[e], __ATOMIC_ACQUIRE,
+                            __HIP_MEMORY_SCOPE_AGENT) == 0u) {
         const uint32_t bits = __hip_atomic_load(
             &args.pmin[e], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         if (bits != kFltMaxBits)
@@ -364,15 +432,24 @@ __global__ void SeedListKernel(const uint32_t* order, Feistel perm,
 
 constexpr int kP = 8;  // slots per wave / points per cooperative group
 
+// Cooperative kernel over `list` (groups of kP entries). Each group's B scan
+// is split into n_segs segments processed by different waves: as many as
+// keep every wave busy, but at least 64 steps each, so a group that breaks
+// early wastes at most n_segs short partial scans.
 template <int ND>
 int LaunchCoop(hd_context* ctx, const float* a, const uint32_t* list,
                const float* list_min, uint32_t n_list, const float* bx,
                const float* by, const float* bz, uint32_t n_steps,
-               unsigned long long* key, unsigned long long* evals) {
+               unsigned long long* key, unsigned long long* evals,
+               uint32_t min_segs = 1) {
   if (n_list == 0) return HD_OK;
-  const uint32_t seg_steps = 64;  // 4096 B points per work item
-  const uint32_t n_segs = (n_steps + seg_steps - 1) / seg_steps;
   const uint32_t n_groups = (n_list + kP - 1) / kP;
+  const uint32_t total_waves =
+      (uint32_t)PersistentBlocks(ctx, kBlock, 8) * (kBlock / kWaveSize);
+  uint32_t n_segs = std::max(min_segs, total_waves / n_groups);
+  n_segs = std::min(n_segs, std::max(1u, n_steps / 64u));
+  const uint32_t seg_steps = (n_steps + n_segs - 1) / n_segs;
+  n_segs = (n_steps + seg_steps - 1) / seg_steps;
   unsigned long long* claim = nullptr;
   uint32_t *broken = nullptr, *pmin = nullptr, *gdone = nullptr;
   HD_TRY(ContextArray(ctx, "coop_claim", 1, &claim));
@@ -401,12 +478,64 @@ int LaunchCoop(hd_context* ctx, const float* a, const uint32_t* list,
   args.gdone = gdone;
   args.key = key;
   args.evals = evals;
-  unsigned long long waves_needed = args.n_items;
   int blocks = PersistentBlocks(ctx, kBlock, 8);
+  const unsigned long long waves_needed = args.n_items;
   if ((unsigned long long)blocks * 4 > waves_needed)
     blocks = (int)std::max<unsigned long long>(1, (waves_needed + 3) / 4);
   EbCoopKernel<ND, kP><<<blocks, kBlock, 0, ctx->stream>>>(args);
   HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
+
+// Slot kernel over order positions [pos_begin, pos_end); escalations are
+// appended to (esc_idx, esc_min) and counted in *esc_count.
+template <int ND>
+int LaunchSlots(hd_context* ctx, const float* a, int order_mode,
+                const uint32_t* order, const float* list_min,
+                const Feistel& perm_a, uint32_t pos_begin, uint32_t pos_end,
+                const float* bx, const float* by, const float* bz,
+                uint32_t n_steps, uint32_t budget_steps,
+                unsigned long long* key, unsigned long long* evals,
+                unsigned int* counters /* cursor, esc_count */,
+                uint32_t* esc_idx, float* esc_min) {
+  if (pos_begin >= pos_end) return HD_OK;
+  HD_TRY(FillU32(ctx, counters, pos_begin, 1));
+  HD_TRY(FillU32(ctx, counters + 1, 0u, 1));
+  SlotArgs sa;
+  sa.a = a;
+  sa.order = order;
+  sa.list_min = list_min;
+  sa.perm_a = perm_a;
+  sa.order_mode = order_mode;
+  sa.pos_end = pos_end;
+  sa.bx = bx;
+  sa.by = by;
+  sa.bz = bz;
+  sa.n_steps = n_steps;
+  sa.budget_steps = budget_steps;
+  sa.key = key;
+  sa.cursor = counters;
+  sa.esc_idx = esc_idx;
+  sa.esc_min = esc_min;
+  sa.esc_count = counters + 1;
+  sa.evals = evals;
+  const uint32_t waves_needed = (pos_end - pos_begin + kP - 1) / kP;
+  int blocks = PersistentBlocks(ctx, kBlock, 8);
+  if ((uint32_t)blocks * 4u > waves_needed)
+    blocks = (int)std::max(1u, (waves_needed + 3u) / 4u);
+  EbSlotKernel<ND, kP><<<blocks, kBlock, 0, ctx->stream>>>(sa);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
+
+int ReadCount(hd_context* ctx, const unsigned int* d_count, uint32_t* out) {
+  void* pinned = nullptr;
+  HD_TRY(ContextPinned(ctx, 64, &pinned));
+  unsigned int* h = static_cast<unsigned int*>(pinned);
+  HD_HIP_TRY(hipMemcpyAsync(h, d_count, 4, hipMemcpyDeviceToHost,
+                            ctx->stream));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  *out = *h;
   return HD_OK;
 }
 
@@ -436,16 +565,17 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
     order_mode = kOrderArray;
   }
 
-  // key, then one evaluation counter per stage (seed, slots, escalation).
+  // key, then one evaluation counter per stage.
   unsigned long long* key = nullptr;
-  HD_TRY(ContextArray(ctx, "eb_key", 4, &key));
-  unsigned long long* ev_seed = key + 1;
-  unsigned long long* ev_slot = key + 2;
-  unsigned long long* ev_esc = key + 3;
-  HD_TRY(FillU64(ctx, key, 0ull, 4));
+  HD_TRY(ContextArray(ctx, "eb_key", 8, &key));
+  // counters: seed, round 1..3 slots, top completions, tail
+  unsigned long long* ev = key + 1;
+  HD_TRY(FillU64(ctx, key, 0ull, 8));
+  unsigned int* counters = nullptr;
+  HD_TRY(ContextArray(ctx, "eb_counters", 4, &counters));
   StageEnd(ctx, st_prep);
 
-  // Stage 1: seed.
+  // Stage 1: exact minima of the first s0 points of the order.
   const uint32_t s0 = std::min<uint32_t>(
       n_a, in.seed_points ? in.seed_points : 256u);
   uint32_t* seed_list = nullptr;
@@ -455,84 +585,96 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
       order, perm_a, order_mode, s0, seed_list);
   HD_HIP_TRY(hipGetLastError());
   HD_TRY((LaunchCoop<ND>(ctx, in.a, seed_list, nullptr, s0, bx, by, bz,
-                         n_steps, key, ev_seed)));
+                         n_steps, key, ev + 0)));
   StageEnd(ctx, st_seed);
 
-  // Stage 2: slots over the remaining order positions.
-  uint32_t n_esc = 0;
+  // Rounds: slots with a growing step budget over the points still open;
+  // after each round the escalated points are sorted by partial minimum
+  // (largest first) and the top ones are completed cooperatively, which
+  // raises cmax2 to an attained value before the next, longer round.
+  // Entries whose partial minimum is already below cmax2 are skipped.
+  static const uint32_t kBudgets[] = {16u, 256u, 4096u};
+  const int n_rounds = in.slot_budget_steps ? 1 : 3;
+  uint64_t round_points[3] = {0, 0, 0};
+  uint64_t n_tail = 0, n_top_total = 0;
   if (s0 < n_a) {
-    unsigned int *cursor = nullptr, *esc_count = nullptr;
-    uint32_t* esc_idx = nullptr;
-    float* esc_min = nullptr;
-    HD_TRY(ContextArray(ctx, "eb_counters", 2, &cursor));
-    esc_count = cursor + 1;
-    HD_TRY(ContextArray(ctx, "eb_esc_idx", n_a, &esc_idx));
-    HD_TRY(ContextArray(ctx, "eb_esc_min", n_a, &esc_min));
-    HD_TRY(FillU32(ctx, cursor, s0, 1));
-    HD_TRY(FillU32(ctx, esc_count, 0u, 1));
-    SlotArgs sa;
-    sa.a = in.a;
-    sa.order = order;
-    sa.perm_a = perm_a;
-    sa.order_mode = order_mode;
-    sa.pos_end = n_a;
-    sa.bx = bx;
-    sa.by = by;
-    sa.bz = bz;
-    sa.n_steps = n_steps;
-    sa.budget_steps =
-        in.slot_budget_steps ? in.slot_budget_steps : 512u;  // 32K evals
-    sa.key = key;
-    sa.cursor = cursor;
-    sa.esc_idx = esc_idx;
-    sa.esc_min = esc_min;
-    sa.esc_count = esc_count;
-    sa.evals = ev_slot;
-    const uint32_t waves_needed = (n_a - s0 + kP - 1) / kP;
-    int blocks = PersistentBlocks(ctx, kBlock, 8);
-    if ((uint32_t)blocks * 4u > waves_needed)
-      blocks = (int)std::max(1u, (waves_needed + 3u) / 4u);
-    const int st_slot = StageBegin(ctx, "eb_slot");
-    EbSlotKernel<ND, kP><<<blocks, kBlock, 0, ctx->stream>>>(sa);
-    HD_HIP_TRY(hipGetLastError());
-    StageEnd(ctx, st_slot);
-
-    // Stage 3: escalated points, most promising first.
-    void* pinned = nullptr;
-    HD_TRY(ContextPinned(ctx, 64, &pinned));
-    unsigned int* h_count = static_cast<unsigned int*>(pinned);
-    HD_HIP_TRY(hipMemcpyAsync(h_count, esc_count, 4, hipMemcpyDeviceToHost,
-                              ctx->stream));
-    HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    n_esc = *h_count;
-    if (n_esc > 0) {
-      float* sorted_min = nullptr;
-      uint32_t* sorted_idx = nullptr;
-      HD_TRY(ContextArray(ctx, "eb_esc_min_sorted", n_esc, &sorted_min));
-      HD_TRY(ContextArray(ctx, "eb_esc_idx_sorted", n_esc, &sorted_idx));
+    uint32_t *idx_in = nullptr, *idx_out = nullptr, *idx_sorted = nullptr;
+    float *min_in = nullptr, *min_out = nullptr, *min_sorted = nullptr;
+    HD_TRY(ContextArray(ctx, "eb_esc_idx_a", n_a, &idx_out));
+    HD_TRY(ContextArray(ctx, "eb_esc_min_a", n_a, &min_out));
+    HD_TRY(ContextArray(ctx, "eb_esc_idx_b", n_a, &idx_sorted));
+    HD_TRY(ContextArray(ctx, "eb_esc_min_b", n_a, &min_sorted));
+    // Round input: first the order positions [s0, n_a), later the sorted
+    // escalations [k_top, n).
+    int mode = order_mode;
+    const uint32_t* list = order;
+    const float* list_min = nullptr;
+    uint32_t begin = s0, end = n_a;
+    for (int r = 0; r < n_rounds && begin < end; ++r) {
+      const uint32_t budget =
+          in.slot_budget_steps ? in.slot_budget_steps : kBudgets[r];
+      round_points[r] = end - begin;
+      const int st_slot = StageBegin(ctx, r == 0 ? "eb_slot" : "eb_slot_more");
+      HD_TRY(LaunchSlots<ND>(ctx, in.a, mode, list, list_min, perm_a, begin,
+                             end, bx, by, bz, n_steps, budget, key, ev + 1 + r,
+                             counters, idx_out, min_out));
+      StageEnd(ctx, st_slot);
+      uint32_t n_esc = 0;
+      HD_TRY(ReadCount(ctx, counters + 1, &n_esc));
+      if (n_esc == 0) {
+        begin = end = 0;
+        break;
+      }
       const int st_sort = StageBegin(ctx, "eb_sort");
-      HD_TRY(SortPairsDescF32(ctx, esc_min, sorted_min, esc_idx, sorted_idx,
+      HD_TRY(SortPairsDescF32(ctx, min_out, min_sorted, idx_out, idx_sorted,
                               n_esc));
       StageEnd(ctx, st_sort);
-      const int st_esc = StageBegin(ctx, "eb_escalate");
-      HD_TRY((LaunchCoop<ND>(ctx, in.a, sorted_idx, sorted_min, n_esc, bx, by,
-                             bz, n_steps, key, ev_esc)));
-      StageEnd(ctx, st_esc);
+      const uint32_t k_top = std::min<uint32_t>(n_esc, 256u);
+      n_top_total += k_top;
+      const int st_top = StageBegin(ctx, "eb_top");
+      HD_TRY((LaunchCoop<ND>(ctx, in.a, idx_sorted, min_sorted, k_top, bx, by,
+                             bz, n_steps, key, ev + 4)));
+      StageEnd(ctx, st_top);
+      // The sorted escalations become the next round's list; swap buffers
+      // so the next round writes where this round's input was.
+      mode = kOrderArray;
+      list = idx_sorted;
+      list_min = min_sorted;
+      begin = k_top;
+      end = n_esc;
+      std::swap(idx_sorted, idx_in);
+      std::swap(min_sorted, min_in);
+      if (idx_sorted == nullptr) {
+        HD_TRY(ContextArray(ctx, "eb_esc_idx_c", n_a, &idx_sorted));
+        HD_TRY(ContextArray(ctx, "eb_esc_min_c", n_a, &min_sorted));
+      }
+    }
+    if (begin < end) {
+      // What is left needed more than the last budget: cooperative scans,
+      // each group split over at least 16 waves.
+      n_tail = end - begin;
+      const int st_tail = StageBegin(ctx, "eb_tail");
+      HD_TRY((LaunchCoop<ND>(ctx, in.a, list + begin, list_min + begin,
+                             end - begin, bx, by, bz, n_steps, key, ev + 5,
+                             16u)));
+      StageEnd(ctx, st_tail);
     }
   }
   const int st_fin = StageBegin(ctx, "eb_finish");
-  HD_TRY(FinishDirected(ctx, ND, in.a, in.b, in.n_b, key, key + 1, 3, out));
+  HD_TRY(FinishDirected(ctx, ND, in.a, in.b, in.n_b, key, ev, 6, out));
   StageEnd(ctx, st_fin);
   if (ctx->profiling) {
     void* pinned = nullptr;
     HD_TRY(ContextPinned(ctx, 64, &pinned));
     unsigned long long* h = static_cast<unsigned long long*>(pinned);
-    HD_HIP_TRY(hipMemcpyAsync(h, key + 1, 24, hipMemcpyDeviceToHost,
-                              ctx->stream));
+    HD_HIP_TRY(hipMemcpyAsync(h, ev, 48, hipMemcpyDeviceToHost, ctx->stream));
     HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
     StageCount(ctx, "eb_seed", h[0], s0);
-    StageCount(ctx, "eb_slot", h[1], n_a - s0);
-    StageCount(ctx, "eb_escalate", h[2], n_esc);
+    StageCount(ctx, "eb_slot", h[1], round_points[0]);
+    StageCount(ctx, "eb_slot_more", h[2] + h[3],
+               round_points[1] + round_points[2]);
+    StageCount(ctx, "eb_top", h[4], n_top_total);
+    StageCount(ctx, "eb_tail", h[5], n_tail);
   }
   return HD_OK;
 }

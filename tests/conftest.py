@@ -4,6 +4,15 @@ from pathlib import Path
 
 import pytest
 
+# torch ships its own HIP runtime under the same soname as the system one
+# (libamdhip64.so.7). Importing it before libhd.so is loaded makes libhd.so
+# bind to that runtime, so torch tensors and the library share one HIP
+# runtime in this process; the other order would start two runtimes.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
 ROOT = Path(__file__).resolve().parent.parent
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
