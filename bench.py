@@ -103,6 +103,8 @@ def main() -> None:
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--variants", default="grid,lbvh,hybrid",
+                    help="other variants timed on the same workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -142,6 +144,27 @@ def main() -> None:
         except hd.HDError as e:
             log(f"learned order unavailable: {e}")
             ms_learned = None
+
+    # Other variants on the same workload (same bits required).
+    other = {}
+    for name in args.variants.split(","):
+        name = name.strip()
+        if not name or name == "eb":
+            continue
+        v = hd.Variant.from_string(name)
+        try:
+            ms_v, res_v, st_v = time_steps(hd, ctx, v, da, db, args.steps, args.warmup,
+                                           learned=False)
+        except hd.HDError as e:
+            log(f"{name} unavailable: {e}")
+            continue
+        rv = res_v[-1]
+        if (rv.cmax2_bits, rv.argmax, rv.nn) != (r0.cmax2_bits, r0.argmax, r0.nn):
+            raise RuntimeError(f"{name} result differs from early break")
+        log(f"{name}: {ms_v:.3f} ms/step, evals {rv.evals}, stages {json.dumps(st_v)}")
+        other[name] = {"ms_per_step": ms_v, "a_points_per_s": a.shape[0] / (ms_v / 1e3),
+                       "evals": rv.evals,
+                       "stages_ms": {k: x["ms"] / max(1, x["calls"]) for k, x in st_v.items()}}
 
     use_learned = ms_learned is not None
     ms = ms_learned if use_learned else ms_plain
@@ -206,6 +229,7 @@ def main() -> None:
             "evals_plain": r0.evals,
             "evals_learned": r1.evals if r1 is not None else None,
             "stages_ms": {k: v["ms"] / max(1, v["calls"]) for k, v in stages.items()},
+            "other_variants": other,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -3,6 +3,7 @@
 // (src/hd_impl/hausdorff_distance_early_break.h:69-76), with the index of the
 // smallest reference point attaining it.
 #include "hd_device.h"
+#include "hd_grid.h"
 #include "hd_internal.h"
 #include "hd_primitives.h"
 
@@ -71,6 +72,9 @@ int NearestNeighbors(hd_context* ctx, int index, int n_dims, const float* q,
   if (index == HD_VARIANT_EB)
     return BruteNearestNeighbors(ctx, n_dims, q, n_q, ref, n_ref, out_d2,
                                  out_idx);
+  if (index == HD_VARIANT_GRID)
+    return GridNearestNeighbors(ctx, n_dims, q, n_q, ref, n_ref, out_d2,
+                                out_idx);
   return Fail(HD_EUNSUPPORTED, "index %d not built yet", index);
 }
 

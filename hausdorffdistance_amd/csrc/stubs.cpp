@@ -2,9 +2,6 @@
 #include "hd_internal.h"
 
 namespace hd {
-int RunGrid(hd_context*, const DirectedArgs&, hd_directed_result*) {
-  return Fail(HD_EUNSUPPORTED, "grid variant not built yet");
-}
 int RunLbvh(hd_context*, const DirectedArgs&, hd_directed_result*) {
   return Fail(HD_EUNSUPPORTED, "lbvh variant not built yet");
 }

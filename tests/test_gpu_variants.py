@@ -1,0 +1,112 @@
+"""GPU parity of the index-based variants (grid, LBVH, hybrid) and of the
+exact nearest-neighbour entry point, through the C ABI, against the CPU
+oracle: cmax2 bits, arg-max and NN index must be identical."""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import hausdorffdistance_amd as hd
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+VARIANTS = [hd.Variant.GRID, hd.Variant.LBVH, hd.Variant.HYBRID]
+
+
+def run(a, b, variant, symmetric=True, ctx=None, **kw):
+    cfg = hd.RunConfig(variant=variant, n_dims=a.shape[1], repeat=1, symmetric=symmetric, **kw)
+    return hd.run_hausdorff_distance(cfg, a, b, ctx=ctx)
+
+
+def triple(r):
+    return {"cmax2_bits": r.cmax2_bits, "argmax": r.argmax, "nn": r.nn}
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=lambda v: v.name)
+def test_golden_fixtures(ctx, variant):
+    cases = np.load(GOLDEN / "cases.npz")
+    expected = json.loads((GOLDEN / "expected.json").read_text())
+    for name, exp in expected.items():
+        a, b = cases[f"{name}__a"], cases[f"{name}__b"]
+        print(f"[golden] {variant.name} {name} {a.shape} {b.shape}", flush=True)
+        r = run(a, b, variant, ctx=ctx)
+        assert triple(r.ab) == exp["ab"], name
+        assert triple(r.ba) == exp["ba"], name
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=lambda v: v.name)
+@pytest.mark.parametrize("n_dims", [2, 3])
+@pytest.mark.parametrize("na,nb", [(1, 1), (1, 500), (500, 1), (63, 65), (3000, 1000)])
+def test_small_sizes(ctx, variant, n_dims, na, nb):
+    a = hd.gen.uniform(300 + na, na, n_dims)
+    b = hd.gen.uniform(400 + nb, nb, n_dims)
+    print(f"[small] {variant.name} {n_dims}D {na}x{nb}", flush=True)
+    r = run(a, b, variant, ctx=ctx)
+    assert triple(r.ab) == triple(oracle.brute_directed(a, b))
+    assert triple(r.ba) == triple(oracle.brute_directed(b, a))
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=lambda v: v.name)
+def test_gmm_200k_2d(ctx, variant):
+    a, b = hd.gen.gmm_pair(200_000)
+    print(f"[gmm] {variant.name}", flush=True)
+    r = run(a, b, variant, ctx=ctx)
+    o_ab, o_ba = oracle.eb_symmetric(a, b)
+    assert triple(r.ab) == triple(o_ab)
+    assert triple(r.ba) == triple(o_ba)
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=lambda v: v.name)
+def test_sphere_3d(ctx, variant):
+    a = hd.gen.sphere(7, 80_000, 0.005, 0.0)
+    b = hd.gen.sphere(8, 60_000, 0.005, 2.0)
+    print(f"[sphere] {variant.name}", flush=True)
+    r = run(a, b, variant, ctx=ctx)
+    o_ab, o_ba = oracle.eb_symmetric(a, b)
+    assert triple(r.ab) == triple(o_ab)
+    assert triple(r.ba) == triple(o_ba)
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=lambda v: v.name)
+def test_degenerate_and_far_apart(ctx, variant):
+    # All B on a line (zero extent in y), A far away and spread out.
+    b = np.stack([np.linspace(0, 1, 999, dtype=np.float32),
+                  np.zeros(999, np.float32)], 1)
+    a = hd.gen.uniform(77, 700, 2) * np.float32(50) + np.float32(10)
+    r = run(a, b, variant, ctx=ctx)
+    assert triple(r.ab) == triple(oracle.brute_directed(a, b))
+    assert triple(r.ba) == triple(oracle.brute_directed(b, a))
+    # Duplicated points everywhere (many exact ties).
+    d = np.repeat(hd.gen.uniform(78, 300, 2), 4, axis=0)
+    e = np.repeat(hd.gen.uniform(79, 200, 2), 3, axis=0)
+    r = run(d, e, variant, ctx=ctx)
+    assert triple(r.ab) == triple(oracle.brute_directed(d, e))
+    assert triple(r.ba) == triple(oracle.brute_directed(e, d))
+
+
+@pytest.mark.parametrize("index", [hd.Variant.EB, hd.Variant.GRID, hd.Variant.LBVH],
+                         ids=lambda v: v.name)
+@pytest.mark.parametrize("n_dims", [2, 3])
+def test_nearest_neighbour_entry_point(ctx, index, n_dims):
+    import torch
+
+    q = hd.gen.uniform(90, 20_000, n_dims)
+    r = hd.gen.uniform(91, 15_000, n_dims)
+    # Exact duplicates in the reference set create index ties.
+    r[1000:1100] = r[2000:2100]
+    q[:50] = r[2000:2050]
+    want_d2, want_idx = oracle.nn_all(q, r)
+    dq = torch.from_numpy(q).cuda()
+    dr = torch.from_numpy(r).cuda()
+    out_d2 = torch.empty(q.shape[0], dtype=torch.float32, device="cuda")
+    out_idx = torch.empty(q.shape[0], dtype=torch.int32, device="cuda")
+    hd.nn_device(ctx, index, dq, dr, out_d2, out_idx)
+    got_d2 = out_d2.cpu().numpy()
+    got_idx = out_idx.cpu().numpy().view(np.uint32)
+    np.testing.assert_array_equal(got_d2.view(np.uint32), want_d2.view(np.uint32))
+    np.testing.assert_array_equal(got_idx, want_idx)
