@@ -1,0 +1,79 @@
+// Software LBVH (lbvh.hip): device records and query arguments.
+#ifndef HD_LBVH_H
+#define HD_LBVH_H
+
+#include <cstdint>
+
+#include "hd_internal.h"
+
+namespace hd {
+
+// Internal node: both child boxes inline (unused z is 0 in 2D) and the two
+// children (bit 31 set: leaf index). 64 bytes, one load per visit.
+struct LbvhNode {
+  float lo[2][3];
+  float hi[2][3];
+  uint32_t child[2];
+  uint32_t pad[2];
+};
+static_assert(sizeof(LbvhNode) == 64, "node record is one 64-byte line");
+
+// Morton quantisation frame.
+struct LbvhFrame {
+  float lo[3];
+  float scale[3];
+  float qmax;
+};
+
+struct LbvhDev {
+  const LbvhNode* nodes;  // n_leaf - 1 internal nodes, root at 0
+  const float* x;         // points in Morton order (SoA)
+  const float* y;
+  const float* z;
+  const uint32_t* orig;   // Morton position -> original index
+  uint32_t n;             // points
+  uint32_t n_leaf;        // leaves (runs of `leaf` points)
+  uint32_t leaf;
+};
+
+struct LbvhIndex {
+  LbvhDev dev;
+  int n_dims = 2;
+};
+
+struct LbvhQueryArgs {
+  LbvhDev tree;
+  int tree_nd;
+  const float* a;          // AoS queries
+  const uint32_t* order;   // order_mode 0
+  Feistel perm;            // order_mode 1
+  int order_mode;          // 0 array, 1 Feistel, 2 identity
+  uint32_t pos_begin, pos_end;
+  int eb;
+  unsigned long long* key;
+  unsigned long long* evals;
+  float* nn_d2;            // optional per-query outputs (by A index)
+  uint32_t* nn_idx;
+  uint32_t* esc_idx;       // traversal-stack overflows
+  float* esc_min;
+  unsigned int* esc_count;
+};
+
+int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
+              uint32_t leaf_size, LbvhIndex* out);
+int LaunchLbvhQuery(hd_context* ctx, const LbvhQueryArgs& args);
+int LbvhNearestNeighbors(hd_context* ctx, int n_dims, const float* q,
+                         uint64_t n_q, const float* ref, uint64_t n_ref,
+                         float* out_d2, uint32_t* out_idx);
+
+// Exact minimum over all of `ref` for the queries listed in `list`
+// (A indices): raises *key with each one's (d2, index) and, when out_d2 is
+// given, writes out_d2[idx] / out_idx[idx] (smallest index among ties).
+int ExactMinForList(hd_context* ctx, int n_dims, const float* q,
+                    const uint32_t* list, uint32_t n_list, const float* ref,
+                    uint64_t n_ref, unsigned long long* key, float* out_d2,
+                    uint32_t* out_idx);
+
+}  // namespace hd
+
+#endif  // HD_LBVH_H

@@ -1,0 +1,587 @@
+// Software LBVH over B for nearest-neighbour queries with early break.
+//
+// The reference answers the same query with OptiX on RT cores: B's grid
+// cells become radius-inflated custom primitives (src/rt/rt_engine.h:956-1180)
+// and an intersection program prunes by cell min/max distance and scans the
+// cell's points with early break (src/rt/shaders/
+// shaders_nn_uniform_grid_2d.cu:16-106). CDNA4 has no ray-tracing units, so
+// here the hierarchy is built and traversed on the shader cores:
+//  * build: 64-bit Morton codes of B (21 bits per axis in 3D, 32 in 2D),
+//    one radix sort, leaves = runs of kLeaf consecutive sorted points,
+//    internal nodes by Karras (HPG 2012) with index tie-breaking for equal
+//    codes, bottom-up boxes with one arrival counter per node;
+//  * node record: both child boxes inline plus the two child ids, so a visit
+//    is one 64-byte load;
+//  * query: one lane per A point, nearest child first, the other pushed on a
+//    per-lane stack in LDS; children whose box is not closer than the best
+//    d2 are pruned; a query stops when its best d2 is strictly below cmax2.
+// Pruning is exact: a box's FP32 squared distance uses the same
+// subtract / square / add sequence as the point d2 and IEEE rounding is
+// monotone, so it never exceeds the d2 of any point inside the box.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "hd_device.h"
+#include "hd_internal.h"
+#include "hd_lbvh.h"
+#include "hd_primitives.h"
+
+namespace hd {
+
+using namespace dev;
+
+namespace {
+
+constexpr uint32_t kLeafFlag = 0x80000000u;
+constexpr int kStack = 64;
+
+HD_DEV unsigned long long Spread2x32(uint32_t v) {
+  unsigned long long x = v;
+  x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+HD_DEV unsigned long long Spread3x21(uint32_t v) {
+  unsigned long long x = v & 0x1FFFFFu;
+  x = (x | (x << 32)) & 0x1F00000000FFFFull;
+  x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+  x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+  x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+template <int ND>
+__global__ void MortonKernel(const float* __restrict__ b, uint32_t n,
+                             LbvhFrame f, unsigned long long* __restrict__ keys,
+                             uint32_t* __restrict__ vals) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    uint32_t q[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const float t = (b[(uint64_t)i * ND + d] - f.lo[d]) * f.scale[d];
+      q[d] = (uint32_t)fminf(fmaxf(t, 0.0f), f.qmax);
+    }
+    unsigned long long code;
+    if constexpr (ND == 2)
+      code = Spread2x32(q[0]) | (Spread2x32(q[1]) << 1);
+    else
+      code = Spread3x21(q[0]) | (Spread3x21(q[1]) << 1) |
+             (Spread3x21(q[2]) << 2);
+    keys[i] = code;
+    vals[i] = i;
+  }
+}
+
+template <int ND>
+__global__ void GatherSortedKernel(const float* __restrict__ b, uint32_t n,
+                                   const uint32_t* __restrict__ vals,
+                                   float* __restrict__ x, float* __restrict__ y,
+                                   float* __restrict__ z,
+                                   uint32_t* __restrict__ orig) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t src = vals[i];
+    const float* p = b + (uint64_t)src * ND;
+    x[i] = p[0];
+    y[i] = p[1];
+    if constexpr (ND == 3) z[i] = p[2];
+    orig[i] = src;
+  }
+}
+
+// Karras' prefix length between leaves i and j (codes of their first
+// points), with the leaf index as tie-breaker for equal codes.
+HD_DEV int Delta(const unsigned long long* codes, int n_leaf, uint32_t leaf,
+                 int i, int j) {
+  if (j < 0 || j >= n_leaf) return -1;
+  const unsigned long long ci = codes[(uint64_t)i * leaf];
+  const unsigned long long cj = codes[(uint64_t)j * leaf];
+  if (ci == cj) return 64 + __clz((uint32_t)(i ^ j));
+  return __clzll(ci ^ cj);
+}
+
+__global__ void KarrasKernel(const unsigned long long* __restrict__ codes,
+                             int n_leaf, uint32_t leaf,
+                             uint32_t* __restrict__ child,
+                             uint32_t* __restrict__ node_parent,
+                             uint32_t* __restrict__ leaf_parent) {
+  const int n_int = n_leaf - 1;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_int;
+       i += gridDim.x * blockDim.x) {
+    const int d =
+        (Delta(codes, n_leaf, leaf, i, i + 1) - Delta(codes, n_leaf, leaf, i, i - 1)) > 0
+            ? 1
+            : -1;
+    const int dmin = Delta(codes, n_leaf, leaf, i, i - d);
+    int lmax = 2;
+    while (Delta(codes, n_leaf, leaf, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+      if (Delta(codes, n_leaf, leaf, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = Delta(codes, n_leaf, leaf, i, j);
+    int s = 0;
+    int t = l;
+    do {
+      t = (t + 1) >> 1;
+      if (Delta(codes, n_leaf, leaf, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + min(d, 0);
+    const bool left_leaf = min(i, j) == gamma;
+    const bool right_leaf = max(i, j) == gamma + 1;
+    child[2 * i] = left_leaf ? ((uint32_t)gamma | kLeafFlag) : (uint32_t)gamma;
+    child[2 * i + 1] =
+        right_leaf ? ((uint32_t)(gamma + 1) | kLeafFlag) : (uint32_t)(gamma + 1);
+    if (left_leaf) leaf_parent[gamma] = i;
+    else node_parent[gamma] = i;
+    if (right_leaf) leaf_parent[gamma + 1] = i;
+    else node_parent[gamma + 1] = i;
+  }
+}
+
+// Leaf boxes, then up the tree: the second child to arrive at a node writes
+// the node record (both child boxes inline) and continues upwards.
+template <int ND>
+__global__ void RefitKernel(int n_leaf, uint32_t leaf, uint32_t n,
+                            const float* __restrict__ x,
+                            const float* __restrict__ y,
+                            const float* __restrict__ z,
+                            const uint32_t* __restrict__ child,
+                            const uint32_t* __restrict__ node_parent,
+                            const uint32_t* __restrict__ leaf_parent,
+                            uint32_t* __restrict__ arrive,
+                            float* __restrict__ leaf_box,
+                            float* __restrict__ node_box,
+                            LbvhNode* __restrict__ nodes) {
+  for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n_leaf;
+       li += gridDim.x * blockDim.x) {
+    float lo[3] = {kFltMax, kFltMax, kFltMax};
+    float hi[3] = {-kFltMax, -kFltMax, -kFltMax};
+    const uint32_t b0 = (uint32_t)li * leaf;
+    const uint32_t b1 = min(b0 + leaf, n);
+    for (uint32_t k = b0; k < b1; ++k) {
+      const float v[3] = {x[k], y[k], ND == 3 ? z[k] : 0.f};
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        lo[d] = fminf(lo[d], v[d]);
+        hi[d] = fmaxf(hi[d], v[d]);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      leaf_box[(uint64_t)li * 6 + d] = lo[d];
+      leaf_box[(uint64_t)li * 6 + 3 + d] = hi[d];
+    }
+    if (n_leaf == 1) continue;
+    uint32_t node = leaf_parent[li];
+    while (true) {
+      // Release this subtree's box, acquire the sibling's.
+      const uint32_t prev = __hip_atomic_fetch_add(
+          &arrive[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (prev == 0) break;  // the sibling is not done yet
+      LbvhNode rec;
+      float ulo[3] = {kFltMax, kFltMax, kFltMax};
+      float uhi[3] = {-kFltMax, -kFltMax, -kFltMax};
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const uint32_t ch = child[2 * node + c];
+        const float* box = (ch & kLeafFlag)
+                               ? &leaf_box[(uint64_t)(ch & ~kLeafFlag) * 6]
+                               : &node_box[(uint64_t)ch * 6];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const float l = d < ND ? box[d] : 0.f;
+          const float h = d < ND ? box[3 + d] : 0.f;
+          rec.lo[c][d] = l;
+          rec.hi[c][d] = h;
+          if (d < ND) {
+            ulo[d] = fminf(ulo[d], l);
+            uhi[d] = fmaxf(uhi[d], h);
+          }
+        }
+        rec.child[c] = ch;
+      }
+      rec.pad[0] = rec.pad[1] = 0u;
+      nodes[node] = rec;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        node_box[(uint64_t)node * 6 + d] = d < ND ? ulo[d] : 0.f;
+        node_box[(uint64_t)node * 6 + 3 + d] = d < ND ? uhi[d] : 0.f;
+      }
+      if (node == 0) break;
+      node = node_parent[node];
+    }
+  }
+}
+
+// FP32 squared distance from q to a box, same operation order as D2.
+template <int ND>
+HD_DEV float BoxD2(const float* q, const float* lo, const float* hi) {
+  float acc = 0.f;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    float diff = 0.f;
+    if (q[d] < lo[d]) diff = lo[d] - q[d];
+    else if (q[d] > hi[d]) diff = q[d] - hi[d];
+    if (d == 0) acc = diff * diff;
+    else acc = acc + diff * diff;
+  }
+  return acc;
+}
+
+template <int ND>
+__global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
+  __shared__ uint32_t stack[kStack * 64];
+  const LbvhDev& t = args.tree;
+  const uint32_t lane = LaneId();
+  const uint32_t lanes_total = gridDim.x * blockDim.x;
+  const uint32_t wave_base = blockIdx.x * blockDim.x;
+  const bool tie_index = args.nn_d2 != nullptr;
+  unsigned long long evals = 0;
+  for (uint32_t base = args.pos_begin + wave_base; base < args.pos_end;
+       base += lanes_total) {
+    const uint32_t pos = base + lane;
+    const bool active = pos < args.pos_end;
+    const float cmax2 = args.eb ? LoadCmax2(args.key) : 0.f;
+    float best = kFltMax;
+    uint32_t best_j = 0xFFFFFFFFu;
+    bool broke = false, overflow = false;
+    uint32_t idx = 0;
+    if (active) {
+      idx = args.order_mode == 1 ? Permute(args.perm, pos)
+            : args.order_mode == 2 ? pos
+                                   : args.order[pos];
+      float q[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+      for (int d = 0; d < ND; ++d) q[d] = args.a[(uint64_t)idx * ND + d];
+      int sp = 0;
+      uint32_t node = t.n_leaf == 1 ? (0u | kLeafFlag) : 0u;
+      while (true) {
+        // Child ids come from the build; an id out of range would mean a
+        // malformed tree, so the query is handed to the exhaustive path
+        // instead of reading outside the arrays.
+        if ((node & kLeafFlag) ? (node & ~kLeafFlag) >= t.n_leaf
+                               : node + 1 >= t.n_leaf) {
+          overflow = true;
+          break;
+        }
+        if (node & kLeafFlag) {
+          const uint32_t li = node & ~kLeafFlag;
+          const uint32_t b0 = li * t.leaf;
+          const uint32_t b1 = min(b0 + t.leaf, t.n);
+          for (uint32_t k = b0; k < b1; ++k) {
+            float d2;
+            if constexpr (ND == 2)
+              d2 = D2(q[0], q[1], t.x[k], t.y[k]);
+            else
+              d2 = D2(q[0], q[1], q[2], t.x[k], t.y[k], t.z[k]);
+            evals++;
+            if (d2 < best) {
+              best = d2;
+              best_j = tie_index ? t.orig[k] : 0u;
+            } else if (tie_index && d2 == best && t.orig[k] < best_j) {
+              best_j = t.orig[k];
+            }
+          }
+          if (args.eb && best < cmax2) {
+            broke = true;
+            break;
+          }
+          if (sp == 0) break;
+          node = stack[(--sp) * 64 + lane];
+          continue;
+        }
+        const LbvhNode rec = t.nodes[node];
+        const float d0 = BoxD2<ND>(q, rec.lo[0], rec.hi[0]);
+        const float d1 = BoxD2<ND>(q, rec.lo[1], rec.hi[1]);
+        const bool v0 = tie_index ? d0 <= best : d0 < best;
+        const bool v1 = tie_index ? d1 <= best : d1 < best;
+        if (v0 && v1) {
+          const bool first0 = d0 <= d1;
+          const uint32_t near = first0 ? rec.child[0] : rec.child[1];
+          const uint32_t far = first0 ? rec.child[1] : rec.child[0];
+          if (sp == kStack) {
+            overflow = true;
+            break;
+          }
+          stack[(sp++) * 64 + lane] = far;
+          node = near;
+        } else if (v0) {
+          node = rec.child[0];
+        } else if (v1) {
+          node = rec.child[1];
+        } else {
+          if (sp == 0) break;
+          node = stack[(--sp) * 64 + lane];
+        }
+      }
+    }
+    if (active && tie_index) {
+      args.nn_d2[idx] = best;
+      args.nn_idx[idx] = overflow ? 0xFFFFFFFEu : best_j;
+    }
+    // A stack overflow hands the point to the exact early-break path.
+    const bool esc = active && overflow;
+    const unsigned long long esc_mask = __ballot(esc);
+    if (esc_mask != 0ull && args.esc_count != nullptr) {
+      uint32_t slot_base = 0;
+      if (lane == 0)
+        slot_base = atomicAdd(args.esc_count, (unsigned)__popcll(esc_mask));
+      slot_base = Uniform(slot_base);
+      if (esc) {
+        const uint32_t rank = __popcll(esc_mask & ((1ull << lane) - 1ull));
+        args.esc_idx[slot_base + rank] = idx;
+        args.esc_min[slot_base + rank] = best;
+      }
+    }
+    if (args.key != nullptr) {
+      const bool done = active && !broke && !overflow && best != kFltMax;
+      const unsigned long long k = done ? PackKey(best, idx) : 0ull;
+      const unsigned long long kmax = WaveMaxU64(k);
+      if (lane == 0 && kmax != 0ull) atomicMax(args.key, kmax);
+    }
+  }
+  unsigned long long e = evals;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+  if (lane == 0 && args.evals != nullptr) atomicAdd(args.evals, e);
+}
+
+}  // namespace
+
+int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
+              uint32_t leaf_size, LbvhIndex* out) {
+  const int nd = n_dims;
+  const uint32_t n = (uint32_t)n_b;
+  const uint32_t leaf = leaf_size ? leaf_size : 8u;
+  float mbr[6];
+  HD_TRY(HostMbr(ctx, nd, b, n_b, mbr));
+  LbvhFrame f;
+  std::memset(&f, 0, sizeof(f));
+  const double levels = nd == 2 ? 4294967295.0 : 2097151.0;
+  f.qmax = (float)levels;
+  for (int d = 0; d < nd; ++d) {
+    f.lo[d] = mbr[d];
+    f.scale[d] = (float)(levels / ((double)mbr[nd + d] - (double)mbr[d]));
+  }
+  unsigned long long *keys = nullptr, *keys_s = nullptr;
+  uint32_t *vals = nullptr, *vals_s = nullptr;
+  HD_TRY(ContextArray(ctx, "lbvh_keys", n, &keys));
+  HD_TRY(ContextArray(ctx, "lbvh_keys_s", n, &keys_s));
+  HD_TRY(ContextArray(ctx, "lbvh_vals", n, &vals));
+  HD_TRY(ContextArray(ctx, "lbvh_vals_s", n, &vals_s));
+  const int blocks = (int)std::min<uint64_t>((n + 255) / 256, 8192);
+  if (nd == 2)
+    MortonKernel<2><<<blocks, 256, 0, ctx->stream>>>(b, n, f, keys, vals);
+  else
+    MortonKernel<3><<<blocks, 256, 0, ctx->stream>>>(b, n, f, keys, vals);
+  HD_HIP_TRY(hipGetLastError());
+  HD_TRY(SortPairsU64(ctx, keys, keys_s, vals, vals_s, n, nd == 2 ? 64 : 63));
+
+  LbvhDev& t = out->dev;
+  std::memset(&t, 0, sizeof(t));
+  float *x = nullptr, *y = nullptr, *z = nullptr;
+  uint32_t* orig = nullptr;
+  HD_TRY(ContextArray(ctx, "lbvh_x", n, &x));
+  HD_TRY(ContextArray(ctx, "lbvh_y", n, &y));
+  if (nd == 3) HD_TRY(ContextArray(ctx, "lbvh_z", n, &z));
+  HD_TRY(ContextArray(ctx, "lbvh_orig", n, &orig));
+  if (nd == 2)
+    GatherSortedKernel<2><<<blocks, 256, 0, ctx->stream>>>(b, n, vals_s, x, y,
+                                                           z, orig);
+  else
+    GatherSortedKernel<3><<<blocks, 256, 0, ctx->stream>>>(b, n, vals_s, x, y,
+                                                           z, orig);
+  HD_HIP_TRY(hipGetLastError());
+
+  const uint32_t n_leaf = (n + leaf - 1) / leaf;
+  const uint32_t n_int = n_leaf > 1 ? n_leaf - 1 : 1;
+  uint32_t *child = nullptr, *node_parent = nullptr, *leaf_parent = nullptr,
+           *arrive = nullptr;
+  float *leaf_box = nullptr, *node_box = nullptr;
+  LbvhNode* nodes = nullptr;
+  HD_TRY(ContextArray(ctx, "lbvh_child", 2 * (uint64_t)n_int, &child));
+  HD_TRY(ContextArray(ctx, "lbvh_node_parent", n_int, &node_parent));
+  HD_TRY(ContextArray(ctx, "lbvh_leaf_parent", n_leaf, &leaf_parent));
+  HD_TRY(ContextArray(ctx, "lbvh_arrive", n_int, &arrive));
+  HD_TRY(ContextArray(ctx, "lbvh_leaf_box", 6 * (uint64_t)n_leaf, &leaf_box));
+  HD_TRY(ContextArray(ctx, "lbvh_node_box", 6 * (uint64_t)n_int, &node_box));
+  HD_TRY(ContextArray(ctx, "lbvh_nodes", n_int, &nodes));
+  HD_TRY(FillU32(ctx, arrive, 0u, n_int));
+  if (n_leaf > 1) {
+    const int kb = (int)std::min<uint64_t>((n_int + 255) / 256, 8192);
+    KarrasKernel<<<kb, 256, 0, ctx->stream>>>(keys_s, (int)n_leaf, leaf, child,
+                                              node_parent, leaf_parent);
+    HD_HIP_TRY(hipGetLastError());
+  }
+  const int lb = (int)std::min<uint64_t>((n_leaf + 255) / 256, 8192);
+  if (nd == 2)
+    RefitKernel<2><<<lb, 256, 0, ctx->stream>>>(
+        (int)n_leaf, leaf, n, x, y, z, child, node_parent, leaf_parent, arrive,
+        leaf_box, node_box, nodes);
+  else
+    RefitKernel<3><<<lb, 256, 0, ctx->stream>>>(
+        (int)n_leaf, leaf, n, x, y, z, child, node_parent, leaf_parent, arrive,
+        leaf_box, node_box, nodes);
+  HD_HIP_TRY(hipGetLastError());
+  t.nodes = nodes;
+  t.x = x;
+  t.y = y;
+  t.z = z;
+  t.orig = orig;
+  t.n = n;
+  t.n_leaf = n_leaf;
+  t.leaf = leaf;
+  out->n_dims = nd;
+  return HD_OK;
+}
+
+int LaunchLbvhQuery(hd_context* ctx, const LbvhQueryArgs& args) {
+  if (args.pos_begin >= args.pos_end) return HD_OK;
+  const uint32_t n = args.pos_end - args.pos_begin;
+  int blocks = ctx->cu_count * 12;
+  blocks = (int)std::min<uint32_t>((uint32_t)blocks, (n + 63) / 64);
+  blocks = std::max(blocks, 1);
+  if (args.tree_nd == 2)
+    LbvhQueryKernel<2><<<blocks, 64, 0, ctx->stream>>>(args);
+  else
+    LbvhQueryKernel<3><<<blocks, 64, 0, ctx->stream>>>(args);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
+
+int LbvhNearestNeighbors(hd_context* ctx, int n_dims, const float* q,
+                         uint64_t n_q, const float* ref, uint64_t n_ref,
+                         float* out_d2, uint32_t* out_idx) {
+  LbvhIndex tree;
+  HD_TRY(BuildLbvh(ctx, n_dims, ref, n_ref, 8, &tree));
+  unsigned int* esc_count = nullptr;
+  uint32_t* esc_idx = nullptr;
+  float* esc_min = nullptr;
+  HD_TRY(ContextArray(ctx, "lbvh_nn_esc_count", 1, &esc_count));
+  HD_TRY(ContextArray(ctx, "lbvh_nn_esc_idx", n_q, &esc_idx));
+  HD_TRY(ContextArray(ctx, "lbvh_nn_esc_min", n_q, &esc_min));
+  HD_TRY(FillU32(ctx, esc_count, 0u, 1));
+  LbvhQueryArgs qa;
+  std::memset(&qa, 0, sizeof(qa));
+  qa.tree = tree.dev;
+  qa.tree_nd = n_dims;
+  qa.a = q;
+  qa.order_mode = 2;
+  qa.pos_begin = 0;
+  qa.pos_end = (uint32_t)n_q;
+  qa.nn_d2 = out_d2;
+  qa.nn_idx = out_idx;
+  qa.esc_idx = esc_idx;
+  qa.esc_min = esc_min;
+  qa.esc_count = esc_count;
+  HD_TRY(LaunchLbvhQuery(ctx, qa));
+  // Queries whose traversal stack overflowed get an exhaustive scan.
+  void* pinned = nullptr;
+  HD_TRY(ContextPinned(ctx, 64, &pinned));
+  unsigned int* h = static_cast<unsigned int*>(pinned);
+  HD_HIP_TRY(hipMemcpyAsync(h, esc_count, 4, hipMemcpyDeviceToHost,
+                            ctx->stream));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return ExactMinForList(ctx, n_dims, q, esc_idx, *h, ref, n_ref, nullptr,
+                         out_d2, out_idx);
+}
+
+namespace {
+
+template <int ND>
+int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
+                hd_directed_result* out) {
+  const uint32_t n_a = (uint32_t)in.n_a;
+  const int st_build = StageBegin(ctx, "lbvh_build");
+  LbvhIndex tree;
+  HD_TRY(BuildLbvh(ctx, ND, in.b, in.n_b, 8, &tree));
+  StageEnd(ctx, st_build);
+
+  unsigned long long* key = nullptr;
+  HD_TRY(ContextArray(ctx, "lbvh_key", 4, &key));
+  HD_TRY(FillU64(ctx, key, 0ull, 4));
+  unsigned int* esc_count = nullptr;
+  uint32_t* esc_idx = nullptr;
+  float* esc_min = nullptr;
+  HD_TRY(ContextArray(ctx, "lbvh_esc_count", 1, &esc_count));
+  HD_TRY(ContextArray(ctx, "lbvh_esc_idx", n_a, &esc_idx));
+  HD_TRY(ContextArray(ctx, "lbvh_esc_min", n_a, &esc_min));
+  HD_TRY(FillU32(ctx, esc_count, 0u, 1));
+
+  LbvhQueryArgs qa;
+  std::memset(&qa, 0, sizeof(qa));
+  qa.tree = tree.dev;
+  qa.tree_nd = ND;
+  qa.a = in.a;
+  const bool shuffle = (in.flags & HD_FLAG_NO_SHUFFLE) == 0;
+  qa.perm = MakeFeistel(n_a, (uint64_t)(int64_t)in.seed, 0);
+  qa.order_mode = shuffle ? 1 : 2;
+  if (in.flags & HD_FLAG_LEARNED_ORDER) {
+    const uint32_t* order = nullptr;
+    HD_TRY(LearnedOrder(ctx, in, &order));
+    qa.order = order;
+    qa.order_mode = 0;
+  }
+  qa.key = key;
+  qa.esc_idx = esc_idx;
+  qa.esc_min = esc_min;
+  qa.esc_count = esc_count;
+
+  // Seed with exact nearest neighbours of the first points of the order,
+  // then early break for the rest.
+  const uint32_t s0 = std::min<uint32_t>(n_a, 65536u);
+  const int st_seed = StageBegin(ctx, "lbvh_seed");
+  qa.pos_begin = 0;
+  qa.pos_end = s0;
+  qa.eb = 0;
+  qa.evals = key + 1;
+  HD_TRY(LaunchLbvhQuery(ctx, qa));
+  StageEnd(ctx, st_seed);
+  const int st_q = StageBegin(ctx, "lbvh_query");
+  qa.pos_begin = s0;
+  qa.pos_end = n_a;
+  qa.eb = 1;
+  qa.evals = key + 2;
+  HD_TRY(LaunchLbvhQuery(ctx, qa));
+  StageEnd(ctx, st_q);
+  // Stack overflows (deep trees) are finished exhaustively.
+  void* pinned = nullptr;
+  HD_TRY(ContextPinned(ctx, 64, &pinned));
+  unsigned int* h = static_cast<unsigned int*>(pinned);
+  HD_HIP_TRY(hipMemcpyAsync(h, esc_count, 4, hipMemcpyDeviceToHost,
+                            ctx->stream));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const uint32_t n_over = *h;
+  HD_TRY(ExactMinForList(ctx, ND, in.a, esc_idx, n_over, in.b, in.n_b, key,
+                         nullptr, nullptr));
+  const int st_fin = StageBegin(ctx, "lbvh_finish");
+  HD_TRY(FinishDirected(ctx, ND, in.a, in.b, in.n_b, key, key + 1, 2, out));
+  StageEnd(ctx, st_fin);
+  if (ctx->profiling) {
+    unsigned long long* hh = static_cast<unsigned long long*>(pinned);
+    HD_HIP_TRY(hipMemcpyAsync(hh, key + 1, 16, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    StageCount(ctx, "lbvh_seed", hh[0], s0);
+    StageCount(ctx, "lbvh_query", hh[1], n_a - s0);
+    StageCount(ctx, "lbvh_overflow", 0, n_over);
+  }
+  return HD_OK;
+}
+
+}  // namespace
+
+int RunLbvh(hd_context* ctx, const DirectedArgs& args,
+            hd_directed_result* out) {
+  if (args.n_dims == 2) return RunLbvhImpl<2>(ctx, args, out);
+  return RunLbvhImpl<3>(ctx, args, out);
+}
+
+}  // namespace hd

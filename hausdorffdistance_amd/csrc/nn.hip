@@ -2,8 +2,11 @@
 // minimum the reference's EB loop computes when it does not break
 // (src/hd_impl/hausdorff_distance_early_break.h:69-76), with the index of the
 // smallest reference point attaining it.
+#include <algorithm>
+
 #include "hd_device.h"
 #include "hd_grid.h"
+#include "hd_lbvh.h"
 #include "hd_internal.h"
 #include "hd_primitives.h"
 
@@ -75,7 +78,79 @@ int NearestNeighbors(hd_context* ctx, int index, int n_dims, const float* q,
   if (index == HD_VARIANT_GRID)
     return GridNearestNeighbors(ctx, n_dims, q, n_q, ref, n_ref, out_d2,
                                 out_idx);
+  if (index == HD_VARIANT_LBVH)
+    return LbvhNearestNeighbors(ctx, n_dims, q, n_q, ref, n_ref, out_d2,
+                                out_idx);
   return Fail(HD_EUNSUPPORTED, "index %d not built yet", index);
+}
+
+}  // namespace hd
+
+namespace hd {
+namespace {
+
+// One wave per listed query: lanes stride over all of ref, then a wave
+// reduction keeps the smallest d2 and, among equal d2, the smallest index.
+template <int ND>
+__global__ __launch_bounds__(256) void ExactMinListKernel(
+    const float* __restrict__ q, const uint32_t* __restrict__ list,
+    uint32_t n_list, const float* __restrict__ r, uint64_t n_r,
+    unsigned long long* key, float* out_d2, uint32_t* out_idx) {
+  const uint32_t lane = dev::LaneId();
+  const uint32_t waves = gridDim.x * (blockDim.x / 64);
+  for (uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) / 64; w < n_list;
+       w += waves) {
+    const uint32_t idx = list[w];
+    float pq[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) pq[d] = q[(uint64_t)idx * ND + d];
+    float best = dev::kFltMax;
+    uint32_t bj = 0xFFFFFFFFu;
+    for (uint64_t j = lane; j < n_r; j += 64) {
+      const float* pr = r + j * ND;
+      const float d = ND == 2 ? dev::D2(pq[0], pq[1], pr[0], pr[1])
+                              : dev::D2(pq[0], pq[1], pq[2], pr[0], pr[1], pr[2]);
+      if (d < best) {
+        best = d;
+        bj = (uint32_t)j;
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const float ob = __shfl_xor(best, off, 64);
+      const uint32_t oj = (uint32_t)__shfl_xor((int)bj, off, 64);
+      if (ob < best || (ob == best && oj < bj)) {
+        best = ob;
+        bj = oj;
+      }
+    }
+    if (lane == 0) {
+      if (out_d2 != nullptr) {
+        out_d2[idx] = best;
+        out_idx[idx] = bj;
+      }
+      if (key != nullptr && best != dev::kFltMax)
+        atomicMax(key, dev::PackKey(best, idx));
+    }
+  }
+}
+
+}  // namespace
+
+int ExactMinForList(hd_context* ctx, int n_dims, const float* q,
+                    const uint32_t* list, uint32_t n_list, const float* ref,
+                    uint64_t n_ref, unsigned long long* key, float* out_d2,
+                    uint32_t* out_idx) {
+  if (n_list == 0) return HD_OK;
+  const uint32_t blocks = std::min<uint32_t>((n_list + 3) / 4, 4096u);
+  if (n_dims == 2)
+    ExactMinListKernel<2><<<blocks, 256, 0, ctx->stream>>>(
+        q, list, n_list, ref, n_ref, key, out_d2, out_idx);
+  else
+    ExactMinListKernel<3><<<blocks, 256, 0, ctx->stream>>>(
+        q, list, n_list, ref, n_ref, key, out_d2, out_idx);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
 }
 
 }  // namespace hd

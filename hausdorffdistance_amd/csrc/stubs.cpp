@@ -2,9 +2,6 @@
 #include "hd_internal.h"
 
 namespace hd {
-int RunLbvh(hd_context*, const DirectedArgs&, hd_directed_result*) {
-  return Fail(HD_EUNSUPPORTED, "lbvh variant not built yet");
-}
 int RunHybrid(hd_context* ctx, const DirectedArgs& a, hd_directed_result* o) {
   return RunEarlyBreak(ctx, a, o);
 }
