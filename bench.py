@@ -133,12 +133,26 @@ def main() -> None:
     ms_learned = None
     st_learned = None
     r1 = None
+    train_s = None
+    model = None
+    final_loss = None
     if not args.no_learned:
         try:
+            log("training the ordering model")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            model = hd.Model.train(ctx, da, db, steps=2000, batch=4096,
+                                   n_samples=min(1_000_000, a.shape[0]))
+            torch.cuda.synchronize()
+            train_s = time.perf_counter() - t0
+            final_loss = model.final_loss
+            log(f"trained in {train_s:.2f} s, final loss {final_loss:.4f}")
             ms_learned, res_learned, st_learned = time_steps(
-                hd, ctx, hd.Variant.EB, da, db, args.steps, args.warmup, learned=True)
+                hd, ctx, hd.Variant.EB, da, db, args.steps, args.warmup, learned=True,
+                model=model)
             r1 = res_learned[-1]
             log(f"learned: {ms_learned:.3f} ms/step, evals {r1.evals}")
+            log(f"learned stages: {json.dumps(st_learned)}")
             if (r1.cmax2_bits, r1.argmax) != (r0.cmax2_bits, r0.argmax):
                 raise RuntimeError("learned order changed the result")
         except hd.HDError as e:
@@ -222,6 +236,8 @@ def main() -> None:
             "plain_value": n_a / (ms_plain / 1e3),
             "learned_ms_per_step": ms_learned,
             "learned_speedup": (ms_plain / ms_learned) if use_learned else None,
+            "model_train_s": train_s,
+            "model_final_loss": final_loss,
             "cmax2_bits": r0.cmax2_bits,
             "hd": r0.hd,
             "argmax": r0.argmax,

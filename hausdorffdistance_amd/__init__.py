@@ -14,12 +14,14 @@ from .api import (  # noqa: F401
     HausdorffDistanceGrid,
     HausdorffDistanceHybrid,
     HausdorffDistanceLbvh,
+    Model,
     Result,
     RunConfig,
     Variant,
     device_count,
     device_info,
     directed_device,
+    mlp_loss_grad,
     nn_device,
     run_hausdorff_distance,
 )

@@ -310,6 +310,80 @@ class HausdorffDistanceHybrid(HausdorffDistance):
     algorithm = "Hybrid"
 
 
+class Model:
+    """Ordering MLP (hd_mlp): d -> 64 -> 64 -> 1, BF16 weights, trained on
+    the GPU on exact NN distances of a subsample of A."""
+
+    def __init__(self, handle: C.c_void_p, n_dims: int, final_loss: float):
+        self._h = handle
+        self.n_dims = n_dims
+        self.final_loss = final_loss
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    @staticmethod
+    def train(ctx: "Context", points_a, points_b, *, steps: int = 2000, batch: int = 4096,
+              n_samples: int = 1_000_000, seed: int = 14, lr: float = 3e-3,
+              label_index: Variant = Variant.GRID) -> "Model":
+        lib = _lib.load()
+        nd = int(points_a.shape[1])
+        pa, na, da, _ = _points(points_a, nd)
+        pb, nb, db, _ = _points(points_b, nd)
+        if not (da and db):
+            raise HDError(_lib.HD_EINVAL, "Model.train needs CUDA tensors")
+        cfg = _lib.MlpTrainConfigC()
+        lib.hd_mlp_train_config_init(C.byref(cfg))
+        cfg.steps, cfg.batch, cfg.n_samples, cfg.seed = steps, batch, n_samples, seed
+        cfg.lr = lr
+        cfg.label_index = int(label_index)
+        h = C.c_void_p()
+        loss = C.c_float(0)
+        _lib.check(lib.hd_mlp_train_device(ctx.handle, nd, pa, na, pb, nb, C.byref(cfg),
+                                           C.byref(h), C.byref(loss)))
+        return Model(h, nd, float(loss.value))
+
+    def predict(self, ctx: "Context", points, out) -> None:
+        p, n, _, _ = _points(points, self.n_dims)
+        _lib.check(_lib.load().hd_mlp_predict_device(ctx.handle, self._h, p, n,
+                                                     C.c_void_p(out.data_ptr())))
+
+    def params(self) -> np.ndarray:
+        lib = _lib.load()
+        out = np.empty(lib.hd_mlp_n_params(self.n_dims), np.float32)
+        _lib.check(lib.hd_mlp_get_params(self._h, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                         None, None))
+        return out
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.load().hd_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mlp_loss_grad(ctx: "Context", params: np.ndarray, x: np.ndarray, y: np.ndarray):
+    """One GPU forward+backward (weights rounded to BF16): (loss, grad)."""
+    lib = _lib.load()
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    nd = int(x.shape[1])
+    g = np.empty_like(p)
+    loss = C.c_float(0)
+    F = C.POINTER(C.c_float)
+    _lib.check(lib.hd_mlp_loss_grad_device(ctx.handle, nd, p.ctypes.data_as(F),
+                                           x.ctypes.data_as(F), y.ctypes.data_as(F),
+                                           int(y.shape[0]), C.byref(loss), g.ctypes.data_as(F)))
+    return float(loss.value), g
+
+
 def device_count() -> int:
     return int(_lib.load().hd_device_count())
 

@@ -29,6 +29,12 @@ __global__ void FillU64Kernel(unsigned long long* p, unsigned long long v,
     p[i] = v;
 }
 
+__global__ void IotaKernel(uint32_t* p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x)
+    p[i] = i;
+}
+
 template <int ND>
 __global__ void GatherSoAKernel(const float* __restrict__ aos, uint32_t n,
                                 uint32_t n_pad, int shuffle, Feistel perm,
@@ -173,6 +179,13 @@ Feistel MakeFeistel(uint32_t n, uint64_t seed, uint32_t stream_id) {
 int FillU32(hd_context* ctx, uint32_t* p, uint32_t value, uint64_t n) {
   if (n == 0) return HD_OK;
   FillU32Kernel<<<GridFor(n, 256, 4096), 256, 0, ctx->stream>>>(p, value, n);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
+
+int IotaU32(hd_context* ctx, uint32_t* p, uint32_t n) {
+  if (n == 0) return HD_OK;
+  IotaKernel<<<GridFor(n, 256, 4096), 256, 0, ctx->stream>>>(p, n);
   HD_HIP_TRY(hipGetLastError());
   return HD_OK;
 }

@@ -539,23 +539,110 @@ int ReadCount(hd_context* ctx, const unsigned int* d_count, uint32_t* out) {
   return HD_OK;
 }
 
+// B repacked for the scans: SoA, pseudo-randomly permuted (unless the
+// caller asks for input order), padded to whole 64-point steps with copies
+// of existing points (duplicates never change a minimum).
+template <int ND>
+int PrepareB(hd_context* ctx, const float* b, uint32_t n_b, bool shuffle,
+             int32_t seed, EbTarget* t) {
+  t->n_bp = (n_b + 63u) & ~63u;
+  t->n_steps = t->n_bp / 64u;
+  t->bx = t->by = t->bz = nullptr;
+  HD_TRY(ContextArray(ctx, "eb_bx", t->n_bp, &t->bx));
+  HD_TRY(ContextArray(ctx, "eb_by", t->n_bp, &t->by));
+  if (ND == 3) HD_TRY(ContextArray(ctx, "eb_bz", t->n_bp, &t->bz));
+  const Feistel perm_b = MakeFeistel(n_b, (uint64_t)(int64_t)seed, 1);
+  return GatherSoA(ctx, ND, b, n_b, t->n_bp, shuffle, perm_b, t->bx, t->by,
+                   t->bz);
+}
+
+// Rounds over the points still open: slots with a growing step budget;
+// after each round the escalated points are sorted by partial minimum
+// (largest first) and the top ones are completed cooperatively, which
+// raises cmax2 to an attained value before the next, longer round. Entries
+// whose partial minimum is already below cmax2 are skipped. ev[0..4]:
+// counters for rounds 1-3, top completions, tail.
+template <int ND>
+int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
+                const EbTarget& t, int order_mode, const uint32_t* order,
+                const float* order_min, const Feistel& perm_a,
+                uint32_t begin, uint32_t end, uint32_t first_budget,
+                unsigned long long* key, unsigned long long* ev,
+                EbRoundStats* rs) {
+  static const uint32_t kBudgets[] = {16u, 256u, 4096u};
+  const int n_rounds = first_budget ? 1 : 3;
+  unsigned int* counters = nullptr;
+  HD_TRY(ContextArray(ctx, "eb_counters", 4, &counters));
+  uint32_t *idx_in = nullptr, *idx_out = nullptr, *idx_sorted = nullptr;
+  float *min_in = nullptr, *min_out = nullptr, *min_sorted = nullptr;
+  HD_TRY(ContextArray(ctx, "eb_esc_idx_a", n_a, &idx_out));
+  HD_TRY(ContextArray(ctx, "eb_esc_min_a", n_a, &min_out));
+  HD_TRY(ContextArray(ctx, "eb_esc_idx_b", n_a, &idx_sorted));
+  HD_TRY(ContextArray(ctx, "eb_esc_min_b", n_a, &min_sorted));
+  int mode = order_mode;
+  const uint32_t* list = order;
+  const float* list_min = order_min;
+  for (int r = 0; r < n_rounds && begin < end; ++r) {
+    const uint32_t budget = first_budget ? first_budget : kBudgets[r];
+    rs->round_points[r] += end - begin;
+    const int st_slot = StageBegin(ctx, r == 0 ? "eb_slot" : "eb_slot_more");
+    HD_TRY(LaunchSlots<ND>(ctx, a, mode, list, list_min, perm_a, begin, end,
+                           t.bx, t.by, t.bz, t.n_steps, budget, key, ev + r,
+                           counters, idx_out, min_out));
+    StageEnd(ctx, st_slot);
+    uint32_t n_esc = 0;
+    HD_TRY(ReadCount(ctx, counters + 1, &n_esc));
+    if (n_esc == 0) {
+      begin = end = 0;
+      break;
+    }
+    const int st_sort = StageBegin(ctx, "eb_sort");
+    HD_TRY(SortPairsDescF32(ctx, min_out, min_sorted, idx_out, idx_sorted,
+                            n_esc));
+    StageEnd(ctx, st_sort);
+    const uint32_t k_top = std::min<uint32_t>(n_esc, 256u);
+    rs->top_points += k_top;
+    const int st_top = StageBegin(ctx, "eb_top");
+    HD_TRY((LaunchCoop<ND>(ctx, a, idx_sorted, min_sorted, k_top, t.bx, t.by,
+                           t.bz, t.n_steps, key, ev + 3)));
+    StageEnd(ctx, st_top);
+    // The sorted escalations become the next round's list; the next round
+    // writes its sort into the buffer this round read from.
+    mode = kOrderArray;
+    list = idx_sorted;
+    list_min = min_sorted;
+    begin = k_top;
+    end = n_esc;
+    std::swap(idx_sorted, idx_in);
+    std::swap(min_sorted, min_in);
+    if (idx_sorted == nullptr) {
+      HD_TRY(ContextArray(ctx, "eb_esc_idx_c", n_a, &idx_sorted));
+      HD_TRY(ContextArray(ctx, "eb_esc_min_c", n_a, &min_sorted));
+    }
+  }
+  if (begin < end) {
+    // What is left needed more than the last budget: cooperative scans,
+    // each group split over at least 16 waves.
+    rs->tail_points += end - begin;
+    const int st_tail = StageBegin(ctx, "eb_tail");
+    HD_TRY((LaunchCoop<ND>(ctx, a, list + begin,
+                           list_min ? list_min + begin : nullptr, end - begin,
+                           t.bx, t.by, t.bz, t.n_steps, key, ev + 4, 16u)));
+    StageEnd(ctx, st_tail);
+  }
+  return HD_OK;
+}
+
 template <int ND>
 int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
                       hd_directed_result* out) {
   const uint32_t n_a = (uint32_t)in.n_a;
   const uint32_t n_b = (uint32_t)in.n_b;
   const bool shuffle = (in.flags & HD_FLAG_NO_SHUFFLE) == 0;
-  const uint32_t n_bp = (n_b + 63u) & ~63u;
-  const uint32_t n_steps = n_bp / 64u;
 
   const int st_prep = StageBegin(ctx, "eb_prep");
-  float *bx = nullptr, *by = nullptr, *bz = nullptr;
-  HD_TRY(ContextArray(ctx, "eb_bx", n_bp, &bx));
-  HD_TRY(ContextArray(ctx, "eb_by", n_bp, &by));
-  if (ND == 3) HD_TRY(ContextArray(ctx, "eb_bz", n_bp, &bz));
-  const Feistel perm_b = MakeFeistel(n_b, (uint64_t)(int64_t)in.seed, 1);
-  HD_TRY(GatherSoA(ctx, ND, in.b, n_b, n_bp, shuffle, perm_b, bx, by, bz));
-
+  EbTarget t;
+  HD_TRY(PrepareB<ND>(ctx, in.b, n_b, shuffle, in.seed, &t));
   // Visit order of A: learned (array), pseudo-random (Feistel) or input.
   const uint32_t* order = nullptr;
   const Feistel perm_a = MakeFeistel(n_a, (uint64_t)(int64_t)in.seed, 0);
@@ -564,15 +651,12 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
     HD_TRY(LearnedOrder(ctx, in, &order));
     order_mode = kOrderArray;
   }
-
   // key, then one evaluation counter per stage.
   unsigned long long* key = nullptr;
   HD_TRY(ContextArray(ctx, "eb_key", 8, &key));
-  // counters: seed, round 1..3 slots, top completions, tail
+  // counters: seed, rounds 1-3, top completions, tail
   unsigned long long* ev = key + 1;
   HD_TRY(FillU64(ctx, key, 0ull, 8));
-  unsigned int* counters = nullptr;
-  HD_TRY(ContextArray(ctx, "eb_counters", 4, &counters));
   StageEnd(ctx, st_prep);
 
   // Stage 1: exact minima of the first s0 points of the order.
@@ -584,82 +668,14 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
   SeedListKernel<<<std::max(1u, (s0 + 255u) / 256u), 256, 0, ctx->stream>>>(
       order, perm_a, order_mode, s0, seed_list);
   HD_HIP_TRY(hipGetLastError());
-  HD_TRY((LaunchCoop<ND>(ctx, in.a, seed_list, nullptr, s0, bx, by, bz,
-                         n_steps, key, ev + 0)));
+  HD_TRY((LaunchCoop<ND>(ctx, in.a, seed_list, nullptr, s0, t.bx, t.by, t.bz,
+                         t.n_steps, key, ev + 0)));
   StageEnd(ctx, st_seed);
 
-  // Rounds: slots with a growing step budget over the points still open;
-  // after each round the escalated points are sorted by partial minimum
-  // (largest first) and the top ones are completed cooperatively, which
-  // raises cmax2 to an attained value before the next, longer round.
-  // Entries whose partial minimum is already below cmax2 are skipped.
-  static const uint32_t kBudgets[] = {16u, 256u, 4096u};
-  const int n_rounds = in.slot_budget_steps ? 1 : 3;
-  uint64_t round_points[3] = {0, 0, 0};
-  uint64_t n_tail = 0, n_top_total = 0;
-  if (s0 < n_a) {
-    uint32_t *idx_in = nullptr, *idx_out = nullptr, *idx_sorted = nullptr;
-    float *min_in = nullptr, *min_out = nullptr, *min_sorted = nullptr;
-    HD_TRY(ContextArray(ctx, "eb_esc_idx_a", n_a, &idx_out));
-    HD_TRY(ContextArray(ctx, "eb_esc_min_a", n_a, &min_out));
-    HD_TRY(ContextArray(ctx, "eb_esc_idx_b", n_a, &idx_sorted));
-    HD_TRY(ContextArray(ctx, "eb_esc_min_b", n_a, &min_sorted));
-    // Round input: first the order positions [s0, n_a), later the sorted
-    // escalations [k_top, n).
-    int mode = order_mode;
-    const uint32_t* list = order;
-    const float* list_min = nullptr;
-    uint32_t begin = s0, end = n_a;
-    for (int r = 0; r < n_rounds && begin < end; ++r) {
-      const uint32_t budget =
-          in.slot_budget_steps ? in.slot_budget_steps : kBudgets[r];
-      round_points[r] = end - begin;
-      const int st_slot = StageBegin(ctx, r == 0 ? "eb_slot" : "eb_slot_more");
-      HD_TRY(LaunchSlots<ND>(ctx, in.a, mode, list, list_min, perm_a, begin,
-                             end, bx, by, bz, n_steps, budget, key, ev + 1 + r,
-                             counters, idx_out, min_out));
-      StageEnd(ctx, st_slot);
-      uint32_t n_esc = 0;
-      HD_TRY(ReadCount(ctx, counters + 1, &n_esc));
-      if (n_esc == 0) {
-        begin = end = 0;
-        break;
-      }
-      const int st_sort = StageBegin(ctx, "eb_sort");
-      HD_TRY(SortPairsDescF32(ctx, min_out, min_sorted, idx_out, idx_sorted,
-                              n_esc));
-      StageEnd(ctx, st_sort);
-      const uint32_t k_top = std::min<uint32_t>(n_esc, 256u);
-      n_top_total += k_top;
-      const int st_top = StageBegin(ctx, "eb_top");
-      HD_TRY((LaunchCoop<ND>(ctx, in.a, idx_sorted, min_sorted, k_top, bx, by,
-                             bz, n_steps, key, ev + 4)));
-      StageEnd(ctx, st_top);
-      // The sorted escalations become the next round's list; swap buffers
-      // so the next round writes where this round's input was.
-      mode = kOrderArray;
-      list = idx_sorted;
-      list_min = min_sorted;
-      begin = k_top;
-      end = n_esc;
-      std::swap(idx_sorted, idx_in);
-      std::swap(min_sorted, min_in);
-      if (idx_sorted == nullptr) {
-        HD_TRY(ContextArray(ctx, "eb_esc_idx_c", n_a, &idx_sorted));
-        HD_TRY(ContextArray(ctx, "eb_esc_min_c", n_a, &min_sorted));
-      }
-    }
-    if (begin < end) {
-      // What is left needed more than the last budget: cooperative scans,
-      // each group split over at least 16 waves.
-      n_tail = end - begin;
-      const int st_tail = StageBegin(ctx, "eb_tail");
-      HD_TRY((LaunchCoop<ND>(ctx, in.a, list + begin, list_min + begin,
-                             end - begin, bx, by, bz, n_steps, key, ev + 5,
-                             16u)));
-      StageEnd(ctx, st_tail);
-    }
-  }
+  EbRoundStats rs;
+  HD_TRY(ProcessOpen<ND>(ctx, in.a, n_a, t, order_mode, order, nullptr, perm_a,
+                         s0, n_a, in.slot_budget_steps, key, ev + 1, &rs));
+
   const int st_fin = StageBegin(ctx, "eb_finish");
   HD_TRY(FinishDirected(ctx, ND, in.a, in.b, in.n_b, key, ev, 6, out));
   StageEnd(ctx, st_fin);
@@ -670,11 +686,11 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
     HD_HIP_TRY(hipMemcpyAsync(h, ev, 48, hipMemcpyDeviceToHost, ctx->stream));
     HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
     StageCount(ctx, "eb_seed", h[0], s0);
-    StageCount(ctx, "eb_slot", h[1], round_points[0]);
+    StageCount(ctx, "eb_slot", h[1], rs.round_points[0]);
     StageCount(ctx, "eb_slot_more", h[2] + h[3],
-               round_points[1] + round_points[2]);
-    StageCount(ctx, "eb_top", h[4], n_top_total);
-    StageCount(ctx, "eb_tail", h[5], n_tail);
+               rs.round_points[1] + rs.round_points[2]);
+    StageCount(ctx, "eb_top", h[4], rs.top_points);
+    StageCount(ctx, "eb_tail", h[5], rs.tail_points);
   }
   return HD_OK;
 }
@@ -685,6 +701,24 @@ int RunEarlyBreak(hd_context* ctx, const DirectedArgs& args,
                   hd_directed_result* out) {
   if (args.n_dims == 2) return RunEarlyBreakImpl<2>(ctx, args, out);
   return RunEarlyBreakImpl<3>(ctx, args, out);
+}
+
+int EbPrepareTarget(hd_context* ctx, int n_dims, const float* b, uint32_t n_b,
+                    bool shuffle, int32_t seed, EbTarget* t) {
+  if (n_dims == 2) return PrepareB<2>(ctx, b, n_b, shuffle, seed, t);
+  return PrepareB<3>(ctx, b, n_b, shuffle, seed, t);
+}
+
+int EbFinishList(hd_context* ctx, int n_dims, const float* a, uint32_t n_a,
+                 const EbTarget& t, const uint32_t* list, const float* list_min,
+                 uint32_t n_list, unsigned long long* key,
+                 unsigned long long* ev, EbRoundStats* rs) {
+  const Feistel none = MakeFeistel(1, 0, 0);
+  if (n_dims == 2)
+    return ProcessOpen<2>(ctx, a, n_a, t, kOrderArray, list, list_min, none, 0,
+                          n_list, 0, key, ev, rs);
+  return ProcessOpen<3>(ctx, a, n_a, t, kOrderArray, list, list_min, none, 0,
+                        n_list, 0, key, ev, rs);
 }
 
 }  // namespace hd

@@ -193,6 +193,12 @@ HD_DEV void ScanCell(const GridDev& g, const float* qf, const double* qd,
   // hold a smaller index, so only strictly farther cells are skipped.
   const double cmin = CellMinDist2<ND>(g, qd, c);
   if (tie_index ? cmin > (double)st.best : cmin >= (double)st.best) return;
+  if (max_cells != 0 && ++st.cells > max_cells) {
+    // Too many cells for this query: hand it to the early-break scan.
+    st.stop = true;
+    st.over = true;
+    return;
+  }
   const uint32_t lin = Linear<ND>(g, c);
   const uint32_t b0 = g.start[lin], b1 = g.end[lin];
   for (uint32_t k = b0; k < b1; ++k) {
@@ -213,10 +219,6 @@ HD_DEV void ScanCell(const GridDev& g, const float* qf, const double* qd,
       st.broke = true;
       return;
     }
-  }
-  if (b1 > b0 && max_cells != 0 && ++st.cells >= max_cells) {
-    st.stop = true;
-    st.over = true;
   }
 }
 
@@ -476,7 +478,7 @@ int LaunchGridQuery(hd_context* ctx, const GridQueryArgs& args) {
 namespace {
 
 template <int ND>
-int RunGridImpl(hd_context* ctx, const DirectedArgs& in,
+int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
                 hd_directed_result* out) {
   const uint32_t n_a = (uint32_t)in.n_a;
   const int st_build = StageBegin(ctx, "grid_build");
@@ -485,11 +487,17 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in,
   StageEnd(ctx, st_build);
 
   unsigned long long* key = nullptr;
-  HD_TRY(ContextArray(ctx, "grid_key", 4, &key));
-  HD_TRY(FillU64(ctx, key, 0ull, 4));
+  HD_TRY(ContextArray(ctx, "grid_key", 8, &key));
+  HD_TRY(FillU64(ctx, key, 0ull, 8));
   unsigned int* esc_count = nullptr;
   HD_TRY(ContextArray(ctx, "grid_esc_count", 1, &esc_count));
   HD_TRY(FillU32(ctx, esc_count, 0u, 1));
+  uint32_t* esc_idx = nullptr;
+  float* esc_min = nullptr;
+  if (max_cells != 0) {
+    HD_TRY(ContextArray(ctx, "grid_esc_idx", n_a, &esc_idx));
+    HD_TRY(ContextArray(ctx, "grid_esc_min", n_a, &esc_min));
+  }
 
   GridQueryArgs qa;
   std::memset(&qa, 0, sizeof(qa));
@@ -507,6 +515,8 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in,
   }
   qa.key = key;
   qa.esc_count = esc_count;
+  qa.esc_idx = esc_idx;
+  qa.esc_min = esc_min;
 
   // Seed: exact nearest neighbours of the first points of the order.
   const uint32_t s0 = std::min<uint32_t>(n_a, 65536u);
@@ -522,22 +532,46 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in,
   qa.pos_begin = s0;
   qa.pos_end = n_a;
   qa.eb = 1;
+  qa.max_cells = max_cells;
   qa.evals = key + 2;
   HD_TRY(LaunchGridQuery(ctx, qa));
   StageEnd(ctx, st_q);
+  uint32_t n_esc = 0;
+  if (max_cells != 0) {
+    // Queries that needed too many cells: exact early-break scans, starting
+    // from the partial minimum the grid found.
+    void* pinned = nullptr;
+    HD_TRY(ContextPinned(ctx, 64, &pinned));
+    unsigned int* h = static_cast<unsigned int*>(pinned);
+    HD_HIP_TRY(hipMemcpyAsync(h, esc_count, 4, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    n_esc = *h;
+    if (n_esc > 0) {
+      const int st_eb = StageBegin(ctx, "hybrid_eb");
+      EbTarget t;
+      HD_TRY(EbPrepareTarget(ctx, ND, in.b, (uint32_t)in.n_b,
+                             (in.flags & HD_FLAG_NO_SHUFFLE) == 0, in.seed, &t));
+      EbRoundStats rs;
+      HD_TRY(EbFinishList(ctx, ND, in.a, n_a, t, esc_idx, esc_min, n_esc, key,
+                          key + 3, &rs));
+      StageEnd(ctx, st_eb);
+    }
+  }
   const int st_fin = StageBegin(ctx, "grid_finish");
-  HD_TRY(FinishDirected(ctx, ND, in.a, in.b, in.n_b, key, key + 1, 2, out));
+  HD_TRY(FinishDirected(ctx, ND, in.a, in.b, in.n_b, key, key + 1, 7, out));
   StageEnd(ctx, st_fin);
   if (ctx->profiling) {
     void* pinned = nullptr;
     HD_TRY(ContextPinned(ctx, 64, &pinned));
     unsigned long long* h = static_cast<unsigned long long*>(pinned);
-    HD_HIP_TRY(hipMemcpyAsync(h, key + 1, 16, hipMemcpyDeviceToHost,
+    HD_HIP_TRY(hipMemcpyAsync(h, key + 1, 56, hipMemcpyDeviceToHost,
                               ctx->stream));
     HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
     StageCount(ctx, "grid_seed", h[0], s0);
     StageCount(ctx, "grid_query", h[1], n_a - s0);
     StageCount(ctx, "grid_build", 0, in.n_b);
+    StageCount(ctx, "hybrid_eb", h[2] + h[3] + h[4] + h[5] + h[6], n_esc);
   }
   return HD_OK;
 }
@@ -546,8 +580,14 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in,
 
 int RunGrid(hd_context* ctx, const DirectedArgs& args,
             hd_directed_result* out) {
-  if (args.n_dims == 2) return RunGridImpl<2>(ctx, args, out);
-  return RunGridImpl<3>(ctx, args, out);
+  if (args.n_dims == 2) return RunGridImpl<2>(ctx, args, 0, out);
+  return RunGridImpl<3>(ctx, args, 0, out);
+}
+
+int RunGridHybrid(hd_context* ctx, const DirectedArgs& args,
+                  uint32_t max_cells, hd_directed_result* out) {
+  if (args.n_dims == 2) return RunGridImpl<2>(ctx, args, max_cells, out);
+  return RunGridImpl<3>(ctx, args, max_cells, out);
 }
 
 int GridNearestNeighbors(hd_context* ctx, int n_dims, const float* q,

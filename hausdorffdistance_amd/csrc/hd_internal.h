@@ -86,6 +86,14 @@ struct hd_context {
   std::vector<hipEvent_t> event_pool;
   std::vector<hd::PendingStage> pending_stages;
   std::map<std::string, hd::StageStat> stage_stats;
+  // Ordering model trained on demand for HD_FLAG_LEARNED_ORDER without an
+  // explicit model, with the inputs it was trained for.
+  // Hybrid selections made by trial (auto_tune), keyed by input signature.
+  std::map<std::string, int> tuned_variant;
+  hd_mlp* cached_model = nullptr;
+  const float* cached_a = nullptr;
+  const float* cached_b = nullptr;
+  uint64_t cached_n_a = 0, cached_n_b = 0;
 };
 
 namespace hd {
@@ -155,6 +163,34 @@ int RunLbvh(hd_context* ctx, const DirectedArgs& args,
             hd_directed_result* out);
 int RunHybrid(hd_context* ctx, const DirectedArgs& args,
               hd_directed_result* out);
+// Grid query with early break whose queries that visit more than max_cells
+// cells are finished by the early-break scan (the reference's hybrid hands
+// such points from the RT query to its EB kernel, hybrid.h:291-295,377-400).
+int RunGridHybrid(hd_context* ctx, const DirectedArgs& args,
+                  uint32_t max_cells, hd_directed_result* out);
+
+// B repacked for the early-break scans (eb.hip).
+struct EbTarget {
+  float* bx = nullptr;
+  float* by = nullptr;
+  float* bz = nullptr;
+  uint32_t n_bp = 0;     // padded to whole 64-point steps
+  uint32_t n_steps = 0;  // n_bp / 64
+};
+struct EbRoundStats {
+  uint64_t round_points[3] = {0, 0, 0};
+  uint64_t top_points = 0;
+  uint64_t tail_points = 0;
+};
+int EbPrepareTarget(hd_context* ctx, int n_dims, const float* b, uint32_t n_b,
+                    bool shuffle, int32_t seed, EbTarget* t);
+// Finishes the listed A points (indices, with optional partial minima over
+// a subset of B) exactly with early break, raising *key; ev[0..4] count
+// evaluations of the rounds, the top completions and the tail.
+int EbFinishList(hd_context* ctx, int n_dims, const float* a, uint32_t n_a,
+                 const EbTarget& t, const uint32_t* list, const float* list_min,
+                 uint32_t n_list, unsigned long long* key,
+                 unsigned long long* ev, EbRoundStats* rs);
 
 // Exact NN of queries against ref by index kind.
 int NearestNeighbors(hd_context* ctx, int index, int n_dims, const float* q,

@@ -23,6 +23,9 @@ int ExclusiveScanU32(hd_context* ctx, const uint32_t* in, uint32_t* out,
 // Host construction of a Feistel permutation of [0, n) (hd_device.h).
 Feistel MakeFeistel(uint32_t n, uint64_t seed, uint32_t stream_id);
 
+// p[i] = i for i < n.
+int IotaU32(hd_context* ctx, uint32_t* p, uint32_t n);
+
 // Fills n 32-bit words with `value` (a kernel, so it is capturable).
 int FillU32(hd_context* ctx, uint32_t* p, uint32_t value, uint64_t n);
 int FillU64(hd_context* ctx, unsigned long long* p, unsigned long long value,

@@ -1,0 +1,108 @@
+// Hybrid variant: picks an execution strategy per input.
+//
+// Reference: HausdorffDistanceHybrid::CalculateDistance
+// (src/hd_impl/hausdorff_distance_hybrid.h:86-409) samples A with the EB
+// kernel, runs EB only when the sampled iteration counts are small
+// (p80 < eb_only_threshold, :171-199), and otherwise builds a grid and an
+// OptiX BVH over it and hands points that exceed max_hit intersections to
+// the EB kernel (:291-295, :377-400); its parameters can come from
+// gradient-boosted trees (:708-778). Here:
+//  * tiny problems (|A||B| <= 2^26) or few queries (|A| <= 1024): EB only;
+//  * 2D: the Morton grid with early break, queries visiting more than
+//    max_hit cells finished by the EB rounds;
+//  * 3D: the LBVH (adaptive to points on surfaces, where a uniform grid
+//    has few, crowded cells);
+//  * auto_tune: each candidate is run once on the actual input and the
+//    fastest is remembered for that input in the context (the reference
+//    trains its selector offline from a parameter sweep,
+//    src/autotune_hausdorff_distance.cu:181-296).
+// Every candidate returns the same bits; only the time differs.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "hd_internal.h"
+
+namespace hd {
+
+namespace {
+
+int RuleBased(const DirectedArgs& a) {
+  const double work = (double)a.n_a * (double)a.n_b;
+  if (work <= 67108864.0 || a.n_a <= 1024) return HD_VARIANT_EB;
+  return a.n_dims == 2 ? HD_VARIANT_GRID : HD_VARIANT_LBVH;
+}
+
+int RunChoice(hd_context* ctx, int choice, const DirectedArgs& a,
+              hd_directed_result* out) {
+  int rc;
+  if (choice == HD_VARIANT_EB) {
+    rc = RunEarlyBreak(ctx, a, out);
+  } else if (choice == HD_VARIANT_GRID) {
+    rc = RunGridHybrid(ctx, a, a.max_hit ? a.max_hit : 256u, out);
+  } else {
+    rc = RunLbvh(ctx, a, out);
+  }
+  out->variant_used = choice;
+  return rc;
+}
+
+std::string Signature(const DirectedArgs& a) {
+  char buf[160];
+  snprintf(buf, sizeof(buf), "%d:%p:%llu:%p:%llu:%u", a.n_dims,
+           (const void*)a.a, (unsigned long long)a.n_a, (const void*)a.b,
+           (unsigned long long)a.n_b, a.flags);
+  return buf;
+}
+
+}  // namespace
+
+int RunHybrid(hd_context* ctx, const DirectedArgs& args,
+              hd_directed_result* out) {
+  if (!args.auto_tune) return RunChoice(ctx, RuleBased(args), args, out);
+  const std::string sig = Signature(args);
+  auto it = ctx->tuned_variant.find(sig);
+  if (it != ctx->tuned_variant.end())
+    return RunChoice(ctx, it->second, args, out);
+  // Trial: time every candidate once (HIP events on the context stream).
+  const int candidates[3] = {HD_VARIANT_GRID, HD_VARIANT_LBVH, HD_VARIANT_EB};
+  float best_ms = 0.f;
+  int best = -1;
+  hipEvent_t e0, e1;
+  HD_HIP_TRY(hipEventCreate(&e0));
+  HD_HIP_TRY(hipEventCreate(&e1));
+  int rc = HD_OK;
+  for (int c : candidates) {
+    // EB on large inputs is only tried when the index variants are not
+    // clearly cheaper: skip it beyond 2^40 pair work.
+    if (c == HD_VARIANT_EB && (double)args.n_a * args.n_b > 1.1e12 && best >= 0)
+      continue;
+    hd_directed_result r;
+    if ((rc = hipEventRecord(e0, ctx->stream)) != hipSuccess) break;
+    rc = RunChoice(ctx, c, args, &r);
+    if (rc != HD_OK) break;
+    if ((rc = hipEventRecord(e1, ctx->stream)) != hipSuccess) break;
+    if ((rc = hipEventSynchronize(e1)) != hipSuccess) break;
+    float ms = 0.f;
+    if ((rc = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+    if (best < 0 || ms < best_ms) {
+      best = c;
+      best_ms = ms;
+      *out = r;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != HD_OK) {
+    if (rc > 0)
+      return Fail(HD_EDEVICE, "auto-tune timing failed: %s",
+                  hipGetErrorString((hipError_t)rc));
+    return rc;
+  }
+  ctx->tuned_variant[sig] = best;
+  out->variant_used = best;
+  return HD_OK;
+}
+
+}  // namespace hd

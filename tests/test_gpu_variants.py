@@ -110,3 +110,22 @@ def test_nearest_neighbour_entry_point(ctx, index, n_dims):
     got_idx = out_idx.cpu().numpy().view(np.uint32)
     np.testing.assert_array_equal(got_d2.view(np.uint32), want_d2.view(np.uint32))
     np.testing.assert_array_equal(got_idx, want_idx)
+
+
+def test_hybrid_auto_tune_keeps_bits(ctx):
+    a, b = hd.gen.gmm_pair(100_000)
+    o_ab, o_ba = oracle.eb_symmetric(a, b)
+    for _ in range(2):  # first call tunes by trial, second uses the choice
+        r = run(a, b, hd.Variant.HYBRID, ctx=ctx, auto_tune=True)
+        assert triple(r.ab) == triple(o_ab)
+        assert triple(r.ba) == triple(o_ba)
+        assert r.ab.variant_used in (0, 1, 2)
+
+
+def test_hybrid_escalates_far_queries(ctx):
+    # B's points far from every A cluster force long ring searches; with a
+    # small max_hit they are finished by the early-break scan.
+    a, b = hd.gen.gmm_pair(60_000)
+    o = oracle.eb_directed(b, a)
+    r = run(b, a, hd.Variant.HYBRID, ctx=ctx, symmetric=False, max_hit=4)
+    assert triple(r.ab) == triple(o)
