@@ -18,6 +18,7 @@ from .api import (  # noqa: F401
     Result,
     RunConfig,
     Variant,
+    batch_directed,
     device_count,
     device_info,
     directed_device,

@@ -310,6 +310,23 @@ class HausdorffDistanceHybrid(HausdorffDistance):
     algorithm = "Hybrid"
 
 
+def batch_directed(ctx: Context, points_a, offs_a, points_b, offs_b) -> list:
+    """Directed h(A_p -> B_p) for every pair p of a batch (CUDA tensors:
+    points (N, d) float32, offsets (n_pairs + 1,) int64 in points). One
+    persistent-kernel launch replayed from a cached HIP graph."""
+    nd = int(points_a.shape[1])
+    pa, _, da, _ = _points(points_a, nd)
+    pb, _, db, _ = _points(points_b, nd)
+    if not (da and db):
+        raise HDError(_lib.HD_EINVAL, "batch_directed needs CUDA tensors")
+    n_pairs = int(offs_a.shape[0]) - 1
+    out = (_lib.DirectedResultC * max(n_pairs, 1))()
+    _lib.check(_lib.load().hd_batch_directed_device(
+        ctx.handle, nd, pa, C.c_void_p(offs_a.data_ptr()), pb, C.c_void_p(offs_b.data_ptr()),
+        n_pairs, 0, out))
+    return [DirectedResult.from_c(out[i]) for i in range(n_pairs)]
+
+
 class Model:
     """Ordering MLP (hd_mlp): d -> 64 -> 64 -> 1, BF16 weights, trained on
     the GPU on exact NN distances of a subsample of A."""

@@ -320,6 +320,7 @@ void hd_context_destroy(hd_context* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->cached_model) hd_mlp_destroy(ctx->cached_model);
+  if (ctx->batch_plan) DestroyBatchPlan(ctx->batch_plan);
   for (auto& kv : ctx->buffers)
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   for (auto& st : ctx->pending_stages) {

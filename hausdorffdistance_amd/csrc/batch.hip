@@ -1,0 +1,455 @@
+// Batched directed Hausdorff distance: many independent (A_p, B_p) pairs of
+// uneven sizes in one persistent-kernel launch, captured in a HIP graph.
+// The reference runs one pair per process (src/main.cpp:148); this is new.
+//
+// Layout: B of every pair repacked into SoA and padded to whole 64-point
+// steps (copies of the pair's own points), pairs laid end to end. Work
+// items are (pair, chunk of A) in longest-first order of estimated cost
+// |A_p| * |B_p|, so the big pairs start first and the small ones fill in
+// at the end. A wave claims an item and runs the same P-slot early-break
+// scan as the single-pair kernel against that pair's B, with the pair's own
+// packed (d2, index) key; B is small enough (< 65536 points) that a slot
+// simply runs to completion or break, no escalation. A second kernel (one
+// wave per pair) names the NN of each pair's arg-max.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hd_device.h"
+#include "hd_internal.h"
+#include "hd_primitives.h"
+
+namespace hd {
+
+using namespace dev;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kP = 8;
+
+struct BatchItem {
+  uint32_t pair;
+  uint32_t a_begin;  // global A index range
+  uint32_t a_end;
+  uint32_t pad;
+};
+
+struct BatchPair {
+  uint32_t b_soa;    // offset of the pair's padded SoA B
+  uint32_t n_steps;  // padded |B_p| / 64
+  uint32_t b_aos;    // offset of the pair's B in the AoS input (points)
+  uint32_t n_b;
+  uint32_t a_aos;    // offset of the pair's A
+  uint32_t n_a;
+};
+
+struct BatchArgs {
+  const float* a;
+  const float* bx;
+  const float* by;
+  const float* bz;
+  const BatchItem* items;
+  uint32_t n_items;
+  const BatchPair* pairs;
+  unsigned long long* keys;  // one per pair
+  unsigned int* claim;
+  unsigned long long* evals;
+};
+
+HD_DEV float ReadLaneF(float v, uint32_t k) {
+  return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), (int)k));
+}
+
+template <int ND>
+__global__ __launch_bounds__(kBlock) void BatchEbKernel(BatchArgs args) {
+  const uint32_t lane = LaneId();
+  const uint32_t wave_id = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  unsigned long long evals = 0;
+  while (true) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(args.claim, 1u);
+    t = Uniform(t);
+    if (t >= args.n_items) break;
+    const BatchItem it = args.items[t];
+    const uint32_t p = Uniform(it.pair);
+    const uint32_t a_begin = Uniform(it.a_begin), a_end = Uniform(it.a_end);
+    const BatchPair pr = args.pairs[p];
+    const uint32_t n_steps = Uniform(pr.n_steps);
+    const float* bx = args.bx + Uniform(pr.b_soa);
+    const float* by = args.by + Uniform(pr.b_soa);
+    const float* bz = ND == 3 ? args.bz + Uniform(pr.b_soa) : nullptr;
+    unsigned long long* key = args.keys + p;
+    uint32_t step = (wave_id * 7919u + t) % n_steps;
+
+    float pa[kP][ND];
+    uint32_t aidx[kP];
+    uint32_t start[kP];
+    float lmin[kP];
+    uint32_t live = 0;
+    uint32_t gstep = 0;
+    float rp[ND];
+    uint32_t next = a_begin;  // next A position to load into the reservoir
+    unsigned long long rmask = 0;
+    uint32_t rbase = 0;
+#pragma unroll
+    for (int q = 0; q < kP; ++q) {
+      lmin[q] = kFltMax;
+      aidx[q] = 0;
+      start[q] = 0;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) pa[q][d] = 0.f;
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) rp[d] = 0.f;
+    float cmax2 = LoadCmax2(key);
+    float b0, b1, b2 = 0.f;
+    {
+      const uint32_t j = (step << 6) + lane;
+      b0 = bx[j];
+      b1 = by[j];
+      if constexpr (ND == 3) b2 = bz[j];
+    }
+    while (true) {
+#pragma unroll
+      for (int q = 0; q < kP; ++q) {
+        if (!(live & (1u << q))) {
+          if (rmask == 0ull && next < a_end) {
+            rbase = next;
+            const uint32_t pos = next + lane;
+            const bool valid = pos < a_end;
+            if (valid) {
+#pragma unroll
+              for (int d = 0; d < ND; ++d) rp[d] = args.a[(uint64_t)pos * ND + d];
+            }
+            rmask = __ballot(valid);
+            next = min(a_end, next + 64u);
+          }
+          if (rmask != 0ull) {
+            const uint32_t k = (uint32_t)__builtin_ctzll(rmask);
+            rmask &= rmask - 1ull;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) pa[q][d] = ReadLaneF(rp[d], k);
+            aidx[q] = rbase + k;
+            lmin[q] = kFltMax;
+            start[q] = gstep;
+            live |= 1u << q;
+          }
+        }
+      }
+      if (live == 0) break;
+      // Prefetch the next step and cmax2 while this step is evaluated.
+      const uint32_t nstep = step + 1 == n_steps ? 0 : step + 1;
+      const uint32_t jn = (nstep << 6) + lane;
+      const float n0 = bx[jn], n1 = by[jn];
+      const float n2 = ND == 3 ? bz[jn] : 0.f;
+      const uint32_t cm_next = __hip_atomic_load(
+          reinterpret_cast<const uint32_t*>(key) + 1, __ATOMIC_RELAXED,
+          __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+      for (int q = 0; q < kP; ++q) {
+        if (live & (1u << q)) {
+          float d;
+          if constexpr (ND == 2)
+            d = D2(pa[q][0], pa[q][1], b0, b1);
+          else
+            d = D2(pa[q][0], pa[q][1], pa[q][2], b0, b1, b2);
+          lmin[q] = fminf(lmin[q], d);
+        }
+      }
+      evals += (unsigned long long)__builtin_popcount(live) * 64ull;
+      step = nstep;
+      ++gstep;
+#pragma unroll
+      for (int q = 0; q < kP; ++q) {
+        if (live & (1u << q)) {
+          if (WaveAny(lmin[q] < cmax2)) {
+            live &= ~(1u << q);
+          } else if (gstep - start[q] >= n_steps) {
+            const float m = WaveMin(lmin[q]);
+            if (lane == 0 && m != kFltMax) atomicMax(key, PackKey(m, aidx[q]));
+            live &= ~(1u << q);
+          }
+        }
+      }
+      b0 = n0;
+      b1 = n1;
+      b2 = n2;
+      cmax2 = __uint_as_float(Uniform(cm_next));
+    }
+  }
+  if (lane == 0) atomicAdd(args.evals, evals);
+}
+
+struct BatchResult {
+  uint32_t cmax2_bits;
+  uint32_t argmax;  // within the pair's A
+  uint32_t nn;      // within the pair's B
+  uint32_t pad;
+};
+
+template <int ND>
+__global__ __launch_bounds__(kBlock) void BatchNnKernel(
+    const float* __restrict__ a, const float* __restrict__ b,
+    const BatchPair* __restrict__ pairs, uint32_t n_pairs,
+    const unsigned long long* __restrict__ keys, BatchResult* __restrict__ out) {
+  const uint32_t lane = LaneId();
+  const uint32_t waves = gridDim.x * (kBlock / 64);
+  for (uint32_t p = (blockIdx.x * kBlock + threadIdx.x) / 64; p < n_pairs;
+       p += waves) {
+    const unsigned long long key = keys[p];
+    const BatchPair pr = pairs[p];
+    BatchResult r;
+    r.cmax2_bits = (uint32_t)(key >> 32);
+    r.pad = 0;
+    if (key == 0ull) {
+      r.argmax = r.nn = 0xFFFFFFFFu;
+    } else {
+      const uint32_t g = 0xFFFFFFFFu - (uint32_t)key;
+      r.argmax = g - pr.a_aos;
+      float pa[ND];
+#pragma unroll
+      for (int d = 0; d < ND; ++d) pa[d] = a[(uint64_t)g * ND + d];
+      uint32_t best = 0xFFFFFFFFu;
+      for (uint32_t j = lane; j < pr.n_b; j += 64) {
+        const float* pb = b + (uint64_t)(pr.b_aos + j) * ND;
+        const float d2 = ND == 2 ? D2(pa[0], pa[1], pb[0], pb[1])
+                                 : D2(pa[0], pa[1], pa[2], pb[0], pb[1], pb[2]);
+        if (__float_as_uint(d2) == r.cmax2_bits) {
+          best = j;
+          break;
+        }
+      }
+      r.nn = WaveMinU32(best);
+    }
+    if (lane == 0) out[p] = r;
+  }
+}
+
+template <int ND>
+__global__ void BatchRepackKernel(const float* __restrict__ b,
+                                  const BatchPair* __restrict__ pairs,
+                                  uint32_t n_pairs, float* __restrict__ x,
+                                  float* __restrict__ y, float* __restrict__ z) {
+  // One block per pair, threads over its padded B.
+  for (uint32_t p = blockIdx.x; p < n_pairs; p += gridDim.x) {
+    const BatchPair pr = pairs[p];
+    const uint32_t n_pad = pr.n_steps * 64u;
+    for (uint32_t j = threadIdx.x; j < n_pad; j += blockDim.x) {
+      const uint32_t src = pr.b_aos + (j < pr.n_b ? j : j % pr.n_b);
+      const float* q = b + (uint64_t)src * ND;
+      x[pr.b_soa + j] = q[0];
+      y[pr.b_soa + j] = q[1];
+      if constexpr (ND == 3) z[pr.b_soa + j] = q[2];
+    }
+  }
+}
+
+__global__ void ZeroKernel(unsigned long long* keys, uint32_t n_keys,
+                           unsigned int* claim, unsigned long long* evals) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_keys;
+       i += gridDim.x * blockDim.x)
+    keys[i] = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *claim = 0u;
+    *evals = 0ull;
+  }
+}
+
+}  // namespace
+
+// Cached plan + graph for one batch signature.
+struct BatchPlan {
+  std::string sig;
+  int n_dims = 2;
+  uint32_t n_pairs = 0;
+  uint32_t n_items = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  BatchResult* h_results = nullptr;  // pinned
+  unsigned long long* h_evals = nullptr;
+};
+
+void DestroyBatchPlan(BatchPlan* p) {
+  if (!p) return;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->h_results) (void)hipHostFree(p->h_results);
+  delete p;
+}
+
+namespace {
+
+template <int ND>
+int BuildPlan(hd_context* ctx, const float* d_a, const uint64_t* d_offs_a,
+              const float* d_b, const uint64_t* d_offs_b, uint32_t n_pairs,
+              uint32_t chunk, BatchPlan* plan) {
+  std::vector<uint64_t> oa(n_pairs + 1), ob(n_pairs + 1);
+  HD_HIP_TRY(hipMemcpy(oa.data(), d_offs_a, sizeof(uint64_t) * (n_pairs + 1),
+                       hipMemcpyDeviceToHost));
+  HD_HIP_TRY(hipMemcpy(ob.data(), d_offs_b, sizeof(uint64_t) * (n_pairs + 1),
+                       hipMemcpyDeviceToHost));
+  std::vector<BatchPair> pairs(n_pairs);
+  uint64_t soa = 0;
+  for (uint32_t p = 0; p < n_pairs; ++p) {
+    if (oa[p + 1] < oa[p] || ob[p + 1] < ob[p])
+      return Fail(HD_EINVAL, "pair %u: offsets decrease", p);
+    const uint64_t na = oa[p + 1] - oa[p], nb = ob[p + 1] - ob[p];
+    if (na == 0 || nb == 0) return Fail(HD_EINVAL, "pair %u is empty", p);
+    if (oa[p + 1] > 0xFFFFFFFEull || ob[p + 1] > 0xFFFFFFFEull)
+      return Fail(HD_EINVAL, "batch exceeds 2^32-2 points");
+    BatchPair& pr = pairs[p];
+    pr.b_soa = (uint32_t)soa;
+    pr.n_steps = (uint32_t)((nb + 63) / 64);
+    pr.b_aos = (uint32_t)ob[p];
+    pr.n_b = (uint32_t)nb;
+    pr.a_aos = (uint32_t)oa[p];
+    pr.n_a = (uint32_t)na;
+    soa += (uint64_t)pr.n_steps * 64;
+    if (soa > 0xFFFFFFFFull) return Fail(HD_EINVAL, "padded batch too large");
+  }
+  // Items: pairs longest first (|A| * |B|), each cut into chunks.
+  std::vector<uint32_t> order(n_pairs);
+  for (uint32_t p = 0; p < n_pairs; ++p) order[p] = p;
+  std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    const double cx = (double)pairs[x].n_a * pairs[x].n_b;
+    const double cy = (double)pairs[y].n_a * pairs[y].n_b;
+    return cx != cy ? cx > cy : x < y;
+  });
+  std::vector<BatchItem> items;
+  for (uint32_t p : order) {
+    const BatchPair& pr = pairs[p];
+    for (uint32_t s = 0; s < pr.n_a; s += chunk)
+      items.push_back({p, pr.a_aos + s, pr.a_aos + std::min(pr.n_a, s + chunk), 0u});
+  }
+  plan->n_items = (uint32_t)items.size();
+
+  BatchItem* d_items = nullptr;
+  BatchPair* d_pairs = nullptr;
+  float *bx = nullptr, *by = nullptr, *bz = nullptr;
+  unsigned long long *keys = nullptr, *evals = nullptr;
+  unsigned int* claim = nullptr;
+  BatchResult* d_res = nullptr;
+  HD_TRY(ContextArray(ctx, "batch_items", items.size(), &d_items));
+  HD_TRY(ContextArray(ctx, "batch_pairs", n_pairs, &d_pairs));
+  HD_TRY(ContextArray(ctx, "batch_bx", soa, &bx));
+  HD_TRY(ContextArray(ctx, "batch_by", soa, &by));
+  if (ND == 3) HD_TRY(ContextArray(ctx, "batch_bz", soa, &bz));
+  HD_TRY(ContextArray(ctx, "batch_keys", n_pairs, &keys));
+  HD_TRY(ContextArray(ctx, "batch_evals", 1, &evals));
+  HD_TRY(ContextArray(ctx, "batch_claim", 1, &claim));
+  HD_TRY(ContextArray(ctx, "batch_res", n_pairs, &d_res));
+  HD_HIP_TRY(hipMemcpy(d_items, items.data(), sizeof(BatchItem) * items.size(),
+                       hipMemcpyHostToDevice));
+  HD_HIP_TRY(hipMemcpy(d_pairs, pairs.data(), sizeof(BatchPair) * n_pairs,
+                       hipMemcpyHostToDevice));
+  HD_HIP_TRY(hipHostMalloc(&plan->h_results,
+                           sizeof(BatchResult) * n_pairs + 64,
+                           hipHostMallocDefault));
+  plan->h_evals = reinterpret_cast<unsigned long long*>(
+      reinterpret_cast<char*>(plan->h_results) + sizeof(BatchResult) * n_pairs);
+
+  // Capture: zero keys, repack B, the persistent scan, the NN pass and the
+  // copy of the results to pinned host memory. No memset nodes.
+  BatchArgs ba;
+  ba.a = d_a;
+  ba.bx = bx;
+  ba.by = by;
+  ba.bz = bz;
+  ba.items = d_items;
+  ba.n_items = plan->n_items;
+  ba.pairs = d_pairs;
+  ba.keys = keys;
+  ba.claim = claim;
+  ba.evals = evals;
+  int blocks = PersistentBlocks(ctx, kBlock, 8);
+  blocks = std::max(1, std::min<int>(blocks, (int)((plan->n_items + 3) / 4)));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HD_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ZeroKernel<<<std::max(1u, std::min(4096u, (n_pairs + 255) / 256)), 256, 0,
+               ctx->stream>>>(keys, n_pairs, claim, evals);
+  BatchRepackKernel<ND><<<std::min(n_pairs, 8192u), 256, 0, ctx->stream>>>(
+      d_b, d_pairs, n_pairs, bx, by, bz);
+  BatchEbKernel<ND><<<blocks, kBlock, 0, ctx->stream>>>(ba);
+  BatchNnKernel<ND><<<std::max(1u, std::min(4096u, (n_pairs + 3) / 4)), kBlock,
+                      0, ctx->stream>>>(d_a, d_b, d_pairs, n_pairs, keys,
+                                        d_res);
+  (void)hipMemcpyAsync(plan->h_results, d_res, sizeof(BatchResult) * n_pairs,
+                       hipMemcpyDeviceToHost, ctx->stream);
+  (void)hipMemcpyAsync(plan->h_evals, evals, 8, hipMemcpyDeviceToHost,
+                       ctx->stream);
+  hipError_t e = hipStreamEndCapture(ctx->stream, &plan->graph);
+  if (e != hipSuccess)
+    return Fail(HD_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+  HD_HIP_TRY(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+  plan->n_dims = ND;
+  plan->n_pairs = n_pairs;
+  return HD_OK;
+}
+
+}  // namespace
+
+}  // namespace hd
+
+using namespace hd;
+
+extern "C" int hd_batch_directed_device(hd_context* ctx, int n_dims,
+                                        const float* d_a,
+                                        const uint64_t* d_offs_a,
+                                        const float* d_b,
+                                        const uint64_t* d_offs_b,
+                                        uint32_t n_pairs, uint32_t flags,
+                                        hd_directed_result* out) {
+  ClearError();
+  if (!ctx || !d_a || !d_b || !d_offs_a || !d_offs_b || !out)
+    return Fail(HD_EINVAL, "null argument");
+  if (n_dims != 2 && n_dims != 3) return Fail(HD_EINVAL, "n_dims must be 2 or 3");
+  if (n_pairs == 0) return Fail(HD_EINVAL, "no pairs");
+  if (flags & ~(HD_FLAG_COUNT_EVALS | HD_FLAG_NO_SHUFFLE))
+    return Fail(HD_EUNSUPPORTED, "batch mode takes COUNT_EVALS only");
+  HD_TRY(ContextActivate(ctx));
+  char sig[192];
+  snprintf(sig, sizeof(sig), "%d:%p:%p:%p:%p:%u", n_dims, (const void*)d_a,
+           (const void*)d_offs_a, (const void*)d_b, (const void*)d_offs_b,
+           n_pairs);
+  BatchPlan* plan = ctx->batch_plan;
+  if (plan == nullptr || plan->sig != sig) {
+    if (plan) DestroyBatchPlan(plan);
+    ctx->batch_plan = nullptr;
+    plan = new BatchPlan();
+    plan->sig = sig;
+    const uint32_t chunk = 256;  // A points per work item
+    const int rc = n_dims == 2 ? BuildPlan<2>(ctx, d_a, d_offs_a, d_b, d_offs_b,
+                                              n_pairs, chunk, plan)
+                               : BuildPlan<3>(ctx, d_a, d_offs_a, d_b, d_offs_b,
+                                              n_pairs, chunk, plan);
+    if (rc != HD_OK) {
+      DestroyBatchPlan(plan);
+      return rc;
+    }
+    ctx->batch_plan = plan;
+  }
+  HD_HIP_TRY(hipEventRecord(ctx->ev_begin, ctx->stream));
+  HD_HIP_TRY(hipGraphLaunch(plan->exec, ctx->stream));
+  HD_HIP_TRY(hipEventRecord(ctx->ev_end, ctx->stream));
+  HD_HIP_TRY(hipEventSynchronize(ctx->ev_end));
+  float ms = 0.f;
+  HD_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
+  for (uint32_t p = 0; p < n_pairs; ++p) {
+    const BatchResult& r = plan->h_results[p];
+    hd_directed_result& o = out[p];
+    std::memset(&o, 0, sizeof(o));
+    o.cmax2_bits = r.cmax2_bits;
+    o.argmax = r.argmax;
+    o.nn = r.nn;
+    o.variant_used = HD_VARIANT_EB;
+    float c;
+    std::memcpy(&c, &r.cmax2_bits, 4);
+    o.hd = std::sqrt(c);
+  }
+  out[0].time_ms = ms;
+  out[0].evals = *plan->h_evals;
+  return HD_OK;
+}

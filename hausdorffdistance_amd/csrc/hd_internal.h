@@ -72,6 +72,11 @@ struct PendingStage {
 
 }  // namespace hd
 
+namespace hd {
+struct BatchPlan;
+void DestroyBatchPlan(BatchPlan* plan);
+}  // namespace hd
+
 // Opaque handle of the C ABI.
 struct hd_context {
   int device = 0;
@@ -90,6 +95,8 @@ struct hd_context {
   // explicit model, with the inputs it was trained for.
   // Hybrid selections made by trial (auto_tune), keyed by input signature.
   std::map<std::string, int> tuned_variant;
+  // Batched-mode plan and HIP graph for the last batch signature.
+  hd::BatchPlan* batch_plan = nullptr;
   hd_mlp* cached_model = nullptr;
   const float* cached_a = nullptr;
   const float* cached_b = nullptr;
