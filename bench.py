@@ -95,6 +95,95 @@ def cpu_baseline(a: np.ndarray, b: np.ndarray, target_s: float = 12.0) -> dict:
     }
 
 
+def secondary(args) -> None:
+    """The other workloads: each variant timed, same bits required; one JSON
+    line with the fastest variant's throughput."""
+    import torch
+
+    import hausdorffdistance_amd as hd
+
+    ctx = hd.Context(0)
+    if args.workload == "batch_4096_2d":
+        log("generating 4096 pairs")
+        bt = hd.gen.batch(9, 4096)
+        da = torch.from_numpy(bt.a).cuda()
+        db = torch.from_numpy(bt.b).cuda()
+        doa = torch.from_numpy(bt.offs_a.astype(np.int64)).cuda()
+        dob = torch.from_numpy(bt.offs_b.astype(np.int64)).cuda()
+        for _ in range(max(1, args.warmup)):
+            hd.batch_directed(ctx, da, doa, db, dob)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = hd.batch_directed(ctx, da, doa, db, dob)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        n_a = int(bt.offs_a[-1])
+        line = {"metric": "batched directed HD, A-points/s over 4096 pairs", "value": n_a / (ms / 1e3),
+                "unit": "A-points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f32", "data": "synthetic",
+                "config": {"workload": "batch_4096_2d: 4096 pairs, sizes log-uniform in [32, 65536), "
+                                       "4-component blobs sigma 0.02, one persistent launch in a HIP graph",
+                           "pairs_per_s": 4096 / (ms / 1e3), "total_a": n_a,
+                           "total_b": int(bt.offs_b[-1]), "evals": res[0].evals,
+                           "graph_ms": res[0].time_ms}}
+        print(json.dumps(line), flush=True)
+        ctx.close()
+        return
+    if args.workload == "uniform_100k_2d":
+        a, b = hd.gen.uniform_pair(100_000, 2)
+        desc = "uniform_100k_2d: A, B 100,000 i.i.d. uniform points in [0,1]^2 (seeds 1, 2)"
+        variants = ["eb", "grid", "lbvh", "hybrid"]
+    elif args.workload == "sphere_20m_15m_3d":
+        log("generating spheres")
+        a, b = hd.gen.sphere_pair()
+        desc = ("sphere_20m_15m_3d: |A| 20M, |B| 15M points on the unit sphere, radial sigma "
+                "0.005, B rotated 2 degrees about z")
+        variants = ["lbvh", "grid", "hybrid"]
+    else:
+        log("generating gmm 50M")
+        a, b = hd.gen.gmm_50m_256()
+        desc = "gmm_50m_256_2d: as gmm_10m_2d with 50M points and 256 components"
+        variants = ["hybrid", "grid", "eb"]
+    da = torch.from_numpy(a).cuda()
+    db = torch.from_numpy(b).cuda()
+    torch.cuda.synchronize()
+    results = {}
+    ref = None
+    for name in variants:
+        v = hd.Variant.from_string(name)
+        ms_v, res_v, st_v = time_steps(hd, ctx, v, da, db, args.steps, args.warmup, learned=False)
+        r = res_v[-1]
+        if ref is None:
+            ref = (r.cmax2_bits, r.argmax, r.nn)
+        elif (r.cmax2_bits, r.argmax, r.nn) != ref:
+            raise RuntimeError(f"{name} differs: {(r.cmax2_bits, r.argmax, r.nn)} vs {ref}")
+        log(f"{name}: {ms_v:.3f} ms/step, evals {r.evals}")
+        results[name] = {"ms_per_step": ms_v, "a_points_per_s": a.shape[0] / (ms_v / 1e3),
+                         "evals": r.evals,
+                         "stages_ms": {k: x["ms"] / max(1, x["calls"]) for k, x in st_v.items()}}
+    best = min(results, key=lambda k: results[k]["ms_per_step"])
+    ms = results[best]["ms_per_step"]
+    line = {"metric": f"A-points/sec, {args.workload} directed HD", "value": a.shape[0] / (ms / 1e3),
+            "unit": "A-points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": desc, "best_variant": best, "variants": results,
+                       "cmax2_bits": ref[0], "argmax": ref[1], "nn": ref[2]}}
+    if args.workload == "uniform_100k_2d" and not args.no_cpu:
+        from oracle import oracle
+        cores = min(os.cpu_count() or 1, 16)
+        o = oracle.eb_directed(a, b, n_threads=cores)
+        if (o.cmax2_bits, o.argmax, o.nn) != ref:
+            raise RuntimeError("GPU result differs from the CPU oracle")
+        line["cpu_baseline"] = {"value": a.shape[0] / (o.compute_ms / 1e3), "unit": "A-points/s",
+                                "cores": int(o.n_threads), "kind": "port",
+                                "sample": f"whole workload, {o.compared_pairs} d2 evaluations"}
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +194,9 @@ def main() -> None:
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--variants", default="grid,lbvh,hybrid",
                     help="other variants timed on the same workload")
+    ap.add_argument("--workload", default="gmm_10m_2d",
+                    choices=["gmm_10m_2d", "uniform_100k_2d", "sphere_20m_15m_3d",
+                             "batch_4096_2d", "gmm_50m_256_2d"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -113,6 +205,10 @@ def main() -> None:
     import torch
 
     import hausdorffdistance_amd as hd
+
+    if args.workload != "gmm_10m_2d":
+        secondary(args)
+        return
 
     log(f"generating gmm {args.n} x {args.n}")
     a, b = hd.gen.gmm_10m(args.n)
@@ -180,6 +276,23 @@ def main() -> None:
                        "evals": rv.evals,
                        "stages_ms": {k: x["ms"] / max(1, x["calls"]) for k, x in st_v.items()}}
 
+    # Symmetric H = max(h(A->B), h(B->A)) with the hybrid selector.
+    sym = None
+    try:
+        ms_ab, res_ab, _ = time_steps(hd, ctx, hd.Variant.HYBRID, da, db, args.steps,
+                                      args.warmup, learned=False)
+        ms_ba, res_ba, st_ba = time_steps(hd, ctx, hd.Variant.HYBRID, db, da, args.steps,
+                                          args.warmup, learned=False)
+        rab, rba = res_ab[-1], res_ba[-1]
+        sym = {"ms": ms_ab + ms_ba, "points_per_s": (a.shape[0] + b.shape[0]) / ((ms_ab + ms_ba) / 1e3),
+               "h_ab": rab.hd, "h_ba": rba.hd, "H": max(rab.hd, rba.hd),
+               "direction": 1 if rba.cmax2_bits > rab.cmax2_bits else 0,
+               "ms_ab": ms_ab, "ms_ba": ms_ba,
+               "ba_stages_ms": {k: x["ms"] / max(1, x["calls"]) for k, x in st_ba.items()}}
+        log(f"symmetric hybrid: {sym['ms']:.3f} ms (A->B {ms_ab:.3f}, B->A {ms_ba:.3f}), H={sym['H']}")
+    except hd.HDError as e:
+        log(f"symmetric hybrid unavailable: {e}")
+
     use_learned = ms_learned is not None
     ms = ms_learned if use_learned else ms_plain
     stages = st_learned if use_learned else st_plain
@@ -246,6 +359,7 @@ def main() -> None:
             "evals_learned": r1.evals if r1 is not None else None,
             "stages_ms": {k: v["ms"] / max(1, v["calls"]) for k, v in stages.items()},
             "other_variants": other,
+            "symmetric_hybrid": sym,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

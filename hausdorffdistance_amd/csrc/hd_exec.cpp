@@ -1,0 +1,291 @@
+// hd_exec: command-line driver over the C ABI, with the reference's flag
+// names (src/flags.cc:3-57, parsed into RunConfig at src/main.cpp:47-132).
+// Inputs: the reference's point cache format (".bin": size_t n, then n
+// packed points; src/loaders/loader.h:150-184), OFF vertex lists
+// (loader.h:105-147, numbers read as one stream, n_dims per point), or a
+// named synthetic workload (include/hd_gen.h). Preprocessing -normalize and
+// -translate follow src/loaders/translate_points.h:42-80.
+//
+//   hd_exec -input1 a.bin -input2 b.bin -input_type bin -n_dims 2 \
+//           -variant eb -execution gpu -repeat 5 -json out.json
+//   hd_exec -input_type synthetic -input1 gmm_10m_2d -n_dims 2 -variant hybrid
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "hd.h"
+#include "hd_gen.h"
+
+namespace {
+
+struct Flags {
+  std::map<std::string, std::string> kv;
+  std::string Get(const std::string& k, const std::string& def) const {
+    auto it = kv.find(k);
+    return it == kv.end() ? def : it->second;
+  }
+  bool Bool(const std::string& k, bool def) const {
+    auto it = kv.find(k);
+    if (it == kv.end()) return def;
+    std::string v = it->second;
+    std::transform(v.begin(), v.end(), v.begin(), ::tolower);
+    return v.empty() || v == "true" || v == "1" || v == "yes";
+  }
+  long Int(const std::string& k, long def) const {
+    auto it = kv.find(k);
+    return it == kv.end() ? def : std::strtol(it->second.c_str(), nullptr, 10);
+  }
+  double Dbl(const std::string& k, double def) const {
+    auto it = kv.find(k);
+    return it == kv.end() ? def : std::strtod(it->second.c_str(), nullptr);
+  }
+};
+
+const char* kBoolFlags[] = {"check", "normalize", "symmetric", "learned_order",
+                            "auto_tune", "overwrite", "shuffle"};
+
+bool IsBool(const std::string& k) {
+  for (const char* b : kBoolFlags)
+    if (k == b) return true;
+  return false;
+}
+
+// gflags style: -flag=value, -flag value, --flag, -noflag for booleans.
+bool Parse(int argc, char** argv, Flags* f) {
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    if (a.size() < 2 || a[0] != '-') {
+      fprintf(stderr, "unexpected argument %s\n", argv[i]);
+      return false;
+    }
+    a = a.substr(a[1] == '-' ? 2 : 1);
+    const size_t eq = a.find('=');
+    if (eq != std::string::npos) {
+      f->kv[a.substr(0, eq)] = a.substr(eq + 1);
+    } else if (IsBool(a)) {
+      f->kv[a] = "true";
+    } else if (a.rfind("no", 0) == 0 && IsBool(a.substr(2))) {
+      f->kv[a.substr(2)] = "false";
+    } else if (i + 1 < argc) {
+      f->kv[a] = argv[++i];
+    } else {
+      fprintf(stderr, "flag -%s needs a value\n", a.c_str());
+      return false;
+    }
+  }
+  return true;
+}
+
+bool LoadBin(const std::string& path, int nd, long limit,
+             std::vector<float>* out) {
+  std::ifstream ifs(path, std::ios::binary);
+  if (!ifs) return false;
+  uint64_t n = 0;
+  ifs.read(reinterpret_cast<char*>(&n), sizeof(n));
+  if (!ifs) return false;
+  n = std::min<uint64_t>(n, (uint64_t)limit);
+  out->resize(n * nd);
+  ifs.read(reinterpret_cast<char*>(out->data()), sizeof(float) * n * nd);
+  return (bool)ifs;
+}
+
+bool LoadOff(const std::string& path, int nd, long limit,
+             std::vector<float>* out) {
+  std::ifstream ifs(path);
+  if (!ifs) return false;
+  std::string line;
+  std::getline(ifs, line);
+  if (line.substr(0, 3) != "OFF") return false;
+  line = line.substr(3);
+  if (line.find_first_not_of(" \t\r") == std::string::npos)
+    std::getline(ifs, line);
+  std::istringstream hs(line);
+  uint64_t nv = 0, nf = 0, ne = 0;
+  hs >> nv >> nf >> ne;
+  if (nv == 0) return false;
+  nv = std::min<uint64_t>(nv, (uint64_t)limit);
+  out->resize(nv * nd);
+  for (uint64_t i = 0; i < nv * nd; ++i)
+    if (!(ifs >> (*out)[i])) return false;
+  return true;
+}
+
+bool Synthetic(const std::string& name, bool second, int nd, long limit,
+               std::vector<float>* out) {
+  auto clip = [&](uint64_t n) { return std::min<uint64_t>(n, (uint64_t)limit); };
+  if (name == "uniform_100k_2d" || name == "uniform_100k_3d") {
+    const uint64_t n = clip(100000);
+    out->resize(n * nd);
+    return hd_gen_uniform(second ? 2 : 1, n, nd, out->data()) == HD_OK;
+  }
+  if (name == "gmm_10m_2d" || name == "gmm_50m_256_2d") {
+    if (nd != 2) return false;
+    const bool big = name == "gmm_50m_256_2d";
+    const uint64_t n = clip(big ? 50000000 : 10000000);
+    std::vector<float> a(2 * n), b(2 * n);
+    const int rc = big ? hd_gen_gmm_pair(n, 256, 0.01, 0.002f, 0.001f, n / 100,
+                                         10, 11, 12, 13, a.data(), b.data())
+                       : hd_gen_gmm_pair(n, 64, 0.01, 0.002f, 0.001f, n / 100, 3,
+                                         4, 5, 6, a.data(), b.data());
+    if (rc != HD_OK) return false;
+    *out = second ? b : a;
+    return true;
+  }
+  if (name == "sphere_20m_15m_3d") {
+    if (nd != 3) return false;
+    const uint64_t n = clip(second ? 15000000 : 20000000);
+    out->resize(3 * n);
+    return hd_gen_sphere(second ? 8 : 7, n, 0.005, second ? 2.0 : 0.0,
+                         out->data()) == HD_OK;
+  }
+  return false;
+}
+
+// src/loaders/translate_points.h:42-63
+void Normalize(std::vector<float>& p, int nd) {
+  const size_t n = p.size() / nd;
+  if (n == 0) return;
+  float lo[3], hi[3];
+  for (int d = 0; d < nd; ++d) lo[d] = hi[d] = p[d];
+  for (size_t i = 0; i < n; ++i)
+    for (int d = 0; d < nd; ++d) {
+      lo[d] = std::min(lo[d], p[i * nd + d]);
+      hi[d] = std::max(hi[d], p[i * nd + d]);
+    }
+  float ext = 0;
+  for (int d = 0; d < nd; ++d) ext = std::max(ext, hi[d] - lo[d]);
+  if (ext == 0) ext = 1;
+  for (size_t i = 0; i < n; ++i)
+    for (int d = 0; d < nd; ++d) p[i * nd + d] = (p[i * nd + d] - lo[d]) / ext;
+}
+
+// src/loaders/translate_points.h:65-80 (dimension 0).
+void Translate(std::vector<float>& p, int nd, double ratio) {
+  const size_t n = p.size() / nd;
+  if (n == 0) return;
+  float lo = p[0], hi = p[0];
+  for (size_t i = 0; i < n; ++i) {
+    lo = std::min(lo, p[i * nd]);
+    hi = std::max(hi, p[i * nd]);
+  }
+  for (size_t i = 0; i < n; ++i) p[i * nd] += (float)((hi - lo) * ratio);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Flags f;
+  if (argc == 1 || !Parse(argc, argv, &f)) {
+    fprintf(stderr,
+            "Usage: hd_exec -input1 A -input2 B -input_type bin|off|synthetic "
+            "-n_dims 2|3 [-variant eb|grid|rt|lbvh|hybrid] [-execution gpu] "
+            "[-repeat 5] [-seed 0] [-check] [-symmetric] [-learned_order] "
+            "[-n_points_cell 8] [-max_hit 256] [-auto_tune] [-limit N] "
+            "[-normalize] [-translate r] [-json out.json]\n");
+    return 1;
+  }
+  hd_run_config cfg;
+  hd_run_config_init(&cfg);
+  cfg.n_dims = (int)f.Int("n_dims", 0);
+  if (cfg.n_dims != 2 && cfg.n_dims != 3) {
+    fprintf(stderr, "Wrong number of dimensions, which can only be 2 or 3\n");
+    return 1;
+  }
+  std::string variant = f.Get("variant", "eb");
+  std::transform(variant.begin(), variant.end(), variant.begin(), ::tolower);
+  if (variant == "eb") cfg.variant = HD_VARIANT_EB;
+  else if (variant == "grid") cfg.variant = HD_VARIANT_GRID;
+  else if (variant == "rt" || variant == "lbvh") cfg.variant = HD_VARIANT_LBVH;
+  else if (variant == "hybrid") cfg.variant = HD_VARIANT_HYBRID;
+  else {
+    fprintf(stderr, "Unknown variant: %s\n", variant.c_str());
+    return 1;
+  }
+  std::string exec = f.Get("execution", "gpu");
+  std::transform(exec.begin(), exec.end(), exec.begin(), ::tolower);
+  if (exec == "gpu") cfg.execution = HD_EXEC_GPU;
+  else if (exec == "cpu") cfg.execution = HD_EXEC_CPU;
+  else {
+    fprintf(stderr, "Unknown execution: %s\n", exec.c_str());
+    return 1;
+  }
+  cfg.repeat = (int)f.Int("repeat", 5);
+  cfg.seed = (int)f.Int("seed", 0);
+  cfg.parallelism = (int)f.Int("parallelism", -1);
+  cfg.sample_rate = (float)f.Dbl("sample_rate", 0.01);
+  cfg.n_points_cell = (uint32_t)f.Int("n_points_cell", 8);
+  cfg.max_hit = (uint32_t)f.Int("max_hit", 256);
+  cfg.eb_only_threshold = (uint32_t)f.Int("eb_only_threshold", 0);
+  cfg.auto_tune = f.Bool("auto_tune", false) ? 1 : 0;
+  if (f.Bool("check", false)) cfg.flags |= HD_FLAG_CHECK;
+  if (f.Bool("symmetric", false)) cfg.flags |= HD_FLAG_SYMMETRIC;
+  if (f.Bool("learned_order", false)) cfg.flags |= HD_FLAG_LEARNED_ORDER;
+  if (!f.Bool("shuffle", true)) cfg.flags |= HD_FLAG_NO_SHUFFLE;
+
+  const long limit = f.Int("limit", 0x7FFFFFFFL);
+  const std::string type = f.Get("input_type", "");
+  const std::string in1 = f.Get("input1", ""), in2 = f.Get("input2", "");
+  std::vector<float> a, b;
+  bool ok = false;
+  if (type == "bin") {
+    ok = LoadBin(in1, cfg.n_dims, limit, &a) && LoadBin(in2, cfg.n_dims, limit, &b);
+  } else if (type == "off") {
+    ok = LoadOff(in1, cfg.n_dims, limit, &a) && LoadOff(in2, cfg.n_dims, limit, &b);
+  } else if (type == "synthetic") {
+    const std::string n2 = in2.empty() ? in1 : in2;
+    ok = Synthetic(in1, false, cfg.n_dims, limit, &a) &&
+         Synthetic(n2, true, cfg.n_dims, limit, &b);
+  } else {
+    fprintf(stderr, "Unsupported input type: %s\n", type.c_str());
+    return 1;
+  }
+  if (!ok) {
+    fprintf(stderr, "cannot load inputs %s / %s (%s)\n", in1.c_str(),
+            in2.c_str(), type.c_str());
+    return 1;
+  }
+  if (f.Bool("normalize", false)) {
+    Normalize(a, cfg.n_dims);
+    Normalize(b, cfg.n_dims);
+  }
+  const double tr = f.Dbl("translate", 0.0);
+  if (tr != 0) Translate(b, cfg.n_dims, tr);
+
+  const uint64_t na = a.size() / cfg.n_dims, nb = b.size() / cfg.n_dims;
+  printf("Points A: %llu Points B: %llu\n", (unsigned long long)na,
+         (unsigned long long)nb);
+  hd_context* ctx = nullptr;
+  if (cfg.execution == HD_EXEC_GPU && hd_context_create(0, &ctx) != HD_OK) {
+    fprintf(stderr, "%s\n", hd_last_error());
+    return 1;
+  }
+  hd_result res;
+  std::vector<char> stats(1 << 20);
+  int rc = hd_run_hausdorff_distance(ctx, &cfg, a.data(), na, b.data(), nb,
+                                     &res, stats.data(), stats.size());
+  if (rc != HD_OK && rc != HD_ETRUNC) {
+    fprintf(stderr, "error %d: %s\n", rc, hd_last_error());
+    hd_context_destroy(ctx);
+    return 1;
+  }
+  printf("HausdorffDistance: distance is %.9g (cmax2 bits 0x%08x, argmax %u, "
+         "nn %u, direction %d)\n",
+         res.hd, res.cmax2_bits, res.argmax, res.nn, res.direction);
+  printf("Avg Running Time %.3f ms\n", res.avg_time_ms);
+  if (res.check_pass >= 0)
+    printf("HausdorffDistance is %s\n", res.check_pass ? "checked" : "WRONG");
+  const std::string json = f.Get("json", "");
+  if (!json.empty()) {
+    std::ofstream ofs(json);
+    ofs << stats.data() << "\n";
+  }
+  hd_context_destroy(ctx);
+  return res.check_pass == 0 ? 2 : 0;
+}
