@@ -524,6 +524,8 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
   qa.pos_begin = 0;
   qa.pos_end = s0;
   qa.eb = 0;
+  // Far seed queries (long runs of empty rings) escalate like the rest.
+  qa.max_cells = max_cells ? std::max<uint32_t>(max_cells, 1024u) : 0u;
   qa.evals = key + 1;
   HD_TRY(LaunchGridQuery(ctx, qa));
   StageEnd(ctx, st_seed);

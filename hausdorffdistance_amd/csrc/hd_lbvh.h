@@ -38,6 +38,7 @@ struct LbvhDev {
 
 struct LbvhIndex {
   LbvhDev dev;
+  LbvhFrame frame;
   int n_dims = 2;
 };
 
@@ -62,6 +63,10 @@ struct LbvhQueryArgs {
 int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
               uint32_t leaf_size, LbvhIndex* out);
 int LaunchLbvhQuery(hd_context* ctx, const LbvhQueryArgs& args);
+// Permutation of pts sorted by Morton code in frame f (device array owned
+// by the context).
+int MortonOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
+                const LbvhFrame& f, uint32_t** order_out);
 int LbvhNearestNeighbors(hd_context* ctx, int n_dims, const float* q,
                          uint64_t n_q, const float* ref, uint64_t n_ref,
                          float* out_d2, uint32_t* out_idx);

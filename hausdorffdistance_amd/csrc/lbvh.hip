@@ -355,6 +355,25 @@ __global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
 
 }  // namespace
 
+int MortonOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
+                const LbvhFrame& f, uint32_t** order_out) {
+  unsigned long long *keys = nullptr, *keys_s = nullptr;
+  uint32_t *vals = nullptr, *order = nullptr;
+  HD_TRY(ContextArray(ctx, "lbvh_qkeys", n, &keys));
+  HD_TRY(ContextArray(ctx, "lbvh_qkeys_s", n, &keys_s));
+  HD_TRY(ContextArray(ctx, "lbvh_qvals", n, &vals));
+  HD_TRY(ContextArray(ctx, "lbvh_qorder", n, &order));
+  const int blocks = (int)std::min<uint64_t>((n + 255) / 256, 8192);
+  if (n_dims == 2)
+    MortonKernel<2><<<blocks, 256, 0, ctx->stream>>>(pts, n, f, keys, vals);
+  else
+    MortonKernel<3><<<blocks, 256, 0, ctx->stream>>>(pts, n, f, keys, vals);
+  HD_HIP_TRY(hipGetLastError());
+  HD_TRY(SortPairsU64(ctx, keys, keys_s, vals, order, n, n_dims == 2 ? 64 : 63));
+  *order_out = order;
+  return HD_OK;
+}
+
 int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
               uint32_t leaf_size, LbvhIndex* out) {
   const int nd = n_dims;
@@ -364,7 +383,9 @@ int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
   HD_TRY(HostMbr(ctx, nd, b, n_b, mbr));
   LbvhFrame f;
   std::memset(&f, 0, sizeof(f));
-  const double levels = nd == 2 ? 4294967295.0 : 2097151.0;
+  // Largest quantisation level, exactly representable in FP32 so the
+  // clamp in MortonKernel never converts an out-of-range float.
+  const double levels = nd == 2 ? 4294967040.0 : 2097151.0;
   f.qmax = (float)levels;
   for (int d = 0; d < nd; ++d) {
     f.lo[d] = mbr[d];
@@ -439,6 +460,7 @@ int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
   t.n_leaf = n_leaf;
   t.leaf = leaf;
   out->n_dims = nd;
+  out->frame = f;
   return HD_OK;
 }
 
@@ -534,6 +556,15 @@ int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
   qa.esc_min = esc_min;
   qa.esc_count = esc_count;
 
+  // Visit the early-break queries in Morton order of A (in B's frame), so
+  // lanes of a wave descend the same subtrees: coherent node loads.
+  uint32_t* morton_order = nullptr;
+  if (!(in.flags & HD_FLAG_LEARNED_ORDER)) {
+    const int st_ord = StageBegin(ctx, "lbvh_order");
+    HD_TRY(MortonOrder(ctx, ND, in.a, n_a, tree.frame, &morton_order));
+    StageEnd(ctx, st_ord);
+  }
+
   // Seed with exact nearest neighbours of the first points of the order,
   // then early break for the rest.
   const uint32_t s0 = std::min<uint32_t>(n_a, 65536u);
@@ -545,7 +576,15 @@ int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
   HD_TRY(LaunchLbvhQuery(ctx, qa));
   StageEnd(ctx, st_seed);
   const int st_q = StageBegin(ctx, "lbvh_query");
-  qa.pos_begin = s0;
+  if (morton_order != nullptr) {
+    // All points, spatially ordered; the seed points are recomputed (their
+    // keys are already in, so this only costs their traversals).
+    qa.order = morton_order;
+    qa.order_mode = 0;
+    qa.pos_begin = 0;
+  } else {
+    qa.pos_begin = s0;
+  }
   qa.pos_end = n_a;
   qa.eb = 1;
   qa.evals = key + 2;
