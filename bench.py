@@ -95,6 +95,21 @@ def cpu_baseline(a: np.ndarray, b: np.ndarray, target_s: float = 12.0) -> dict:
     }
 
 
+def copy_probe(mib: int) -> None:
+    """A plain device-to-device copy of known size, to calibrate the HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE) against."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "repeats": 3}), flush=True)
+
+
 def secondary(args) -> None:
     """The other workloads: each variant timed, same bits required; one JSON
     line with the fastest variant's throughput."""
@@ -194,10 +209,17 @@ def main() -> None:
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--variants", default="grid,lbvh,hybrid",
                     help="other variants timed on the same workload")
+    ap.add_argument("--no-symmetric", action="store_true",
+                    help="skip the symmetric hybrid measurement")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 3 times (counter calibration)")
     ap.add_argument("--workload", default="gmm_10m_2d",
                     choices=["gmm_10m_2d", "uniform_100k_2d", "sphere_20m_15m_3d",
                              "batch_4096_2d", "gmm_50m_256_2d"])
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -259,7 +281,7 @@ def main() -> None:
     other = {}
     for name in args.variants.split(","):
         name = name.strip()
-        if not name or name == "eb":
+        if not name or name in ("eb", "none"):
             continue
         v = hd.Variant.from_string(name)
         try:
@@ -279,6 +301,8 @@ def main() -> None:
     # Symmetric H = max(h(A->B), h(B->A)) with the hybrid selector.
     sym = None
     try:
+        if args.no_symmetric:
+            raise hd.HDError(0, "skipped (--no-symmetric)")
         ms_ab, res_ab, _ = time_steps(hd, ctx, hd.Variant.HYBRID, da, db, args.steps,
                                       args.warmup, learned=False)
         ms_ba, res_ba, st_ba = time_steps(hd, ctx, hd.Variant.HYBRID, db, da, args.steps,
@@ -299,14 +323,27 @@ def main() -> None:
     n_a = a.shape[0]
     value = n_a / (ms / 1e3)
 
-    # Roofline of the dominant kernel (slot kernel of the plain order).
-    slot = st_plain.get("eb_slot", {})
-    calls = max(1, slot.get("calls", 1))
-    t_kernel = slot.get("ms", float("nan")) / calls / 1e3
-    evals = slot.get("evals", 0) / calls
-    points = slot.get("points", 0) / calls
-    bytes_alg = 8.0 * (points + evals)
+    # Roofline of the dominant kernel: every EbSlotKernel launch of the
+    # plain-order steps (the three slot rounds), HIP events on the library's
+    # stream around each launch.
+    slot_ms = 0.0
+    slot_launches = 0
+    slot_evals = 0
+    slot_points = 0
+    for name in ("eb_slot", "eb_slot_more"):
+        st = st_plain.get(name, {})
+        slot_ms += st.get("ms", 0.0)
+        slot_launches += st.get("calls", 0)
+        slot_evals += st.get("evals", 0)
+        slot_points += st.get("points", 0)
+    launches = max(1, slot_launches)
+    t_kernel = slot_ms / launches / 1e3
+    bytes_alg = 8.0 * (slot_points + slot_evals) / launches
     achieved = bytes_alg / t_kernel / 1e9 if t_kernel > 0 else None
+    traffic = None
+    traffic_file = ROOT / "profiles" / "traffic_EbSlotKernel_gmm_10m_2d.json"
+    if traffic_file.exists():
+        traffic = json.loads(traffic_file.read_text()).get("hbm_bytes_per_launch")
     roofline = {
         "bound": "hbm",
         "kernel": "EbSlotKernel<2,8>",
@@ -314,10 +351,11 @@ def main() -> None:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS if achieved else None,
-        "traffic": None,
+        "traffic": traffic,
         "algorithmic_bytes": bytes_alg,
         "kernel_ms": t_kernel * 1e3,
-        "valu_frac": (evals * LANE_OPS_PER_EVAL[2] / t_kernel / VALU_LANE_OPS)
+        "launches_per_step": slot_launches / max(1, args.steps),
+        "valu_frac": (slot_evals / launches * LANE_OPS_PER_EVAL[2] / t_kernel / VALU_LANE_OPS)
         if t_kernel > 0 else None,
     }
 
