@@ -132,3 +132,33 @@ def test_generators_are_deterministic_and_follow_their_recipes():
     assert np.float32((x >> 40) * 2.0 ** -24) == a1[0, 0]
     sub = hd.gen.subsample(14, 1000, 50)
     assert len(set(sub.tolist())) == 50 and sub.max() < 1000
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors have the size and field offsets the C compiler
+    gives the header's structs."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"hd_run_config": _lib.RunConfigC, "hd_directed_result": _lib.DirectedResultC,
+               "hd_result": _lib.ResultC, "hd_mlp_train_config": _lib.MlpTrainConfigC}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "hd.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c99", f"-I{INCLUDE}", str(src), "-o", str(exe)], check=True)
+    got = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    want = []
+    for cname, py in structs.items():
+        want.append(f"{cname} size {C.sizeof(py)}")
+        for fname, _ in py._fields_:
+            want.append(f"{cname} {fname} {getattr(py, fname).offset}")
+    assert [g for g in got if g] == want
