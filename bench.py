@@ -33,6 +33,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # VALU lane-op ceiling: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz.
 VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
 LANE_OPS_PER_EVAL = {2: 6, 3: 9}
+# The early-break scan kernel and the pipeline stages that launch it.
+SCAN_KERNEL = "EbScanKernel"
+SCAN_KERNEL_FULL = "EbScanKernel<2,8,4>"
+SCAN_STAGES = ("eb_seed", "eb_slot", "eb_slot_more", "eb_top", "eb_tail")
 
 
 def log(msg: str) -> None:
@@ -323,30 +327,32 @@ def main() -> None:
     n_a = a.shape[0]
     value = n_a / (ms / 1e3)
 
-    # Roofline of the dominant kernel: every EbSlotKernel launch of the
-    # plain-order steps (the three slot rounds), HIP events on the library's
-    # stream around each launch.
-    slot_ms = 0.0
-    slot_launches = 0
-    slot_evals = 0
-    slot_points = 0
-    for name in ("eb_slot", "eb_slot_more"):
+    # Roofline of the dominant kernel: every EbScanKernel launch of the
+    # plain-order steps (seed, the three budgeted rounds, the top completions
+    # and the tail), HIP events on the library's stream around each stage.
+    scan_ms = 0.0
+    scan_launches = 0
+    scan_evals = 0
+    scan_points = 0
+    for name in SCAN_STAGES:
         st = st_plain.get(name, {})
-        slot_ms += st.get("ms", 0.0)
-        slot_launches += st.get("calls", 0)
-        slot_evals += st.get("evals", 0)
-        slot_points += st.get("points", 0)
-    launches = max(1, slot_launches)
-    t_kernel = slot_ms / launches / 1e3
-    bytes_alg = 8.0 * (slot_points + slot_evals) / launches
+        scan_ms += st.get("ms", 0.0)
+        scan_launches += st.get("calls", 0)
+        scan_evals += st.get("evals", 0)
+        scan_points += st.get("points", 0)
+    launches = max(1, scan_launches)
+    t_kernel = scan_ms / launches / 1e3
+    bytes_alg = 8.0 * (scan_points + scan_evals) / launches
     achieved = bytes_alg / t_kernel / 1e9 if t_kernel > 0 else None
     traffic = None
-    traffic_file = ROOT / "profiles" / "traffic_EbSlotKernel_gmm_10m_2d.json"
+    traffic_file = ROOT / "profiles" / f"traffic_{SCAN_KERNEL}_gmm_10m_2d.json"
     if traffic_file.exists():
-        traffic = json.loads(traffic_file.read_text()).get("hbm_bytes_per_launch")
+        tr = json.loads(traffic_file.read_text())
+        if tr.get("kernel") == SCAN_KERNEL_FULL:
+            traffic = tr.get("hbm_bytes_per_launch")
     roofline = {
         "bound": "hbm",
-        "kernel": "EbSlotKernel<2,8>",
+        "kernel": SCAN_KERNEL_FULL,
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -354,8 +360,8 @@ def main() -> None:
         "traffic": traffic,
         "algorithmic_bytes": bytes_alg,
         "kernel_ms": t_kernel * 1e3,
-        "launches_per_step": slot_launches / max(1, args.steps),
-        "valu_frac": (slot_evals / launches * LANE_OPS_PER_EVAL[2] / t_kernel / VALU_LANE_OPS)
+        "launches_per_step": scan_launches / max(1, args.steps),
+        "valu_frac": (scan_evals / launches * LANE_OPS_PER_EVAL[2] / t_kernel / VALU_LANE_OPS)
         if t_kernel > 0 else None,
     }
 

@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "hd_device.h"
 #include "hd_internal.h"
@@ -60,24 +61,49 @@ HD_DEV uint32_t OrderAt(int mode, const uint32_t* order, const Feistel& perm,
   return order[pos];
 }
 
-struct SlotArgs {
+// Segment scans. B's steps are split into n_segs segments of seg_steps
+// steps (the last may be shorter). Every segment has its own queue over the
+// same list of order positions; a wave works on one segment at a time and
+// keeps P slots, each holding one A point in scalar registers. The wave's
+// lanes stream the segment's B steps cyclically (64 coalesced SoA points per
+// step) and every B point in registers is tested against all P slots, so a
+// slot that joins at any step has covered the whole segment seg_len steps
+// later. Per slot:
+//  * its running minimum falls strictly below cmax2: dropped (cannot raise
+//    the maximum);
+//  * n_segs == 1 and it scanned all of B: its exact minimum goes into the
+//    packed key;
+//  * n_segs == 1 and it used budget_steps first: escalated with its partial
+//    minimum (esc_idx / esc_min);
+//  * n_segs > 1 and it scanned its segment: the segment minimum goes into
+//    pmin[pos] (atomic min) and gdone[pos] counts it; the wave completing the
+//    last segment of an entry no segment dropped publishes pmin[pos];
+//  * n_segs > 1 and another segment dropped the entry (broken[pos], polled
+//    every 16 steps): dropped.
+struct ScanArgs {
   const float* a;         // AoS A
   const uint32_t* order;  // visit order (kOrderArray)
   const float* list_min;  // optional partial minima per position: a position
                           // whose value is already below cmax2 is skipped
   Feistel perm_a;
   int order_mode;         // kOrder*
-  uint32_t pos_end;       // order positions [cursor start, pos_end)
+  uint32_t pos_end;       // positions [cursor start, pos_end)
   const float* bx;
   const float* by;
   const float* bz;
   uint32_t n_steps;       // n_bp / 64
-  uint32_t budget_steps;  // steps before a slot escalates
+  uint32_t seg_steps;     // steps per segment
+  uint32_t n_segs;        // segments; 1 = whole-B scans
+  uint32_t budget_steps;  // n_segs == 1: steps before a slot escalates
   unsigned long long* key;
-  unsigned int* cursor;   // next order position
+  unsigned int* cursors;  // per segment: next order position
+  uint32_t claim;         // positions taken per cursor claim (<= 64)
   uint32_t* esc_idx;      // escalated A indices
   float* esc_min;         // their partial minima
   unsigned int* esc_count;
+  uint32_t* broken;       // n_segs > 1, per position: 1 once dropped
+  uint32_t* pmin;         // n_segs > 1, per position: FP32 bits of the min
+  uint32_t* gdone;        // n_segs > 1, per position: segments completed
   unsigned long long* evals;
 };
 
@@ -96,74 +122,136 @@ HD_DEV uint32_t ReadLaneU(uint32_t v, uint32_t k) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k);
 }
 
-// B step j (64 SoA points) into registers.
+// B step `step` (64 SoA points) into registers. `step` is wave-uniform:
+// each row is addressed through a scalar buffer descriptor (64 floats) and
+// the lane's byte offset, so no per-lane address arithmetic is spent.
+HD_DEV float LoadRow(const float* base, uint64_t row, uint32_t lane4) {
+  __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(base + row), (short)0, 256, 0x00020000);
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)lane4, 0, 0));
+}
+
 template <int ND>
 HD_DEV void LoadStep(const float* __restrict__ bx, const float* __restrict__ by,
                      const float* __restrict__ bz, uint32_t step,
                      uint32_t lane, float* b) {
-  const uint32_t j = (step << 6) + lane;
-  b[0] = bx[j];
-  b[1] = by[j];
-  if constexpr (ND == 3) b[2] = bz[j];
+  const uint64_t row = (uint64_t)Uniform(step) << 6;
+  const uint32_t lane4 = lane << 2;
+  b[0] = LoadRow(bx, row, lane4);
+  b[1] = LoadRow(by, row, lane4);
+  if constexpr (ND == 3) b[2] = LoadRow(bz, row, lane4);
 }
 
-template <int ND, int P>
+// Running minima are FP32 bit patterns (MinBits).
+template <int ND, int P, bool kAllLive>
 HD_DEV void EvalStep(const float (&pa)[P][ND], uint32_t live_mask,
-                     const float* b, float (&lmin)[P]) {
+                     const float* b, uint32_t (&lmin)[P]) {
 #pragma unroll
   for (int p = 0; p < P; ++p) {
-    if (live_mask & (1u << p)) {
+    if (kAllLive || (live_mask & (1u << p))) {
       float d;
       if constexpr (ND == 2)
         d = D2(pa[p][0], pa[p][1], b[0], b[1]);
       else
         d = D2(pa[p][0], pa[p][1], pa[p][2], b[0], b[1], b[2]);
-      lmin[p] = fminf(lmin[p], d);
+      lmin[p] = MinBits(lmin[p], __float_as_uint(d));
     }
   }
 }
 
-HD_DEV uint32_t NextStep(uint32_t s, uint32_t n) { return s + 1 == n ? 0 : s + 1; }
-
-// Stage 2. One wave = P slots; block = 4 independent waves.
-// Refill goes through a per-wave reservoir: 64 order positions claimed with
-// one atomic, each lane loading one candidate point in parallel; slots then
-// take points from the reservoir with readlane.
-template <int ND, int P>
-__global__ __launch_bounds__(kBlock) void EbSlotKernel(SlotArgs args) {
-  const uint32_t lane = LaneId();
-  const uint32_t wave_id = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-  const uint32_t n_steps = args.n_steps;
-  uint32_t step = (uint32_t)(((uint64_t)wave_id * 7919u) % n_steps);
-
+// Slot state of one wave over one segment; S = steps per iteration.
+template <int ND, int P, int S>
+struct ScanWave {
+  static constexpr uint32_t kAll = (1u << P) - 1u;
   float pa[P][ND];
   uint32_t aidx[P];
+  uint32_t lpos[P];
   uint32_t start[P];
-  float lmin[P];
-  uint32_t live = 0;  // bit p: slot p holds a point
+  uint32_t lmin[P];
+  uint32_t lane_pos;  // lane p < P: list position of slot p
+  uint32_t live;
+  float bcur[S][ND];
+  uint32_t step;
+  uint32_t gstep;
+  unsigned long long evals;
+};
+
+HD_DEV uint32_t NextInSeg(uint32_t st, uint32_t seg_begin, uint32_t seg_end) {
+  return st + 1 == seg_end ? seg_begin : st + 1;
+}
+
+// Steps a wave's slots S B steps at a time until a slot breaks (hit), is
+// flagged by another segment (gone), or reaches next_due.
+template <int ND, int P, int S, bool kAllLive>
+HD_DEV void ScanIterate(const ScanArgs& args, ScanWave<ND, P, S>& w,
+                        uint32_t lane, uint32_t seg_begin, uint32_t seg_end,
+                        bool split, uint32_t next_due, uint32_t& cmax2,
+                        uint32_t& hit, uint32_t& gone) {
+  const uint32_t* cm_ptr = reinterpret_cast<const uint32_t*>(args.key) + 1;
+  while (true) {
+    float bnext[S][ND];
+    uint32_t st = w.step;
 #pragma unroll
-  for (int p = 0; p < P; ++p) {
-    aidx[p] = 0;
-    start[p] = 0;
-    lmin[p] = kFltMax;
+    for (int k = 0; k < S; ++k) st = NextInSeg(st, seg_begin, seg_end);
+    const uint32_t step_next = st;
 #pragma unroll
-    for (int d = 0; d < ND; ++d) pa[p][d] = 0.f;
+    for (int k = 0; k < S; ++k) {
+      LoadStep<ND>(args.bx, args.by, args.bz, st, lane, bnext[k]);
+      st = NextInSeg(st, seg_begin, seg_end);
+    }
+    const uint32_t cm_next =
+        __hip_atomic_load(cm_ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Split scans poll the other segments' verdicts every 16 steps.
+    const bool poll = split && ((w.gstep + S) & 15u) < (uint32_t)S;
+    uint32_t flag = 0;
+    if (poll && lane < (uint32_t)P && ((w.live >> lane) & 1u))
+      flag = __hip_atomic_load(&args.broken[w.lane_pos], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+      EvalStep<ND, P, kAllLive>(w.pa, w.live, w.bcur[k], w.lmin);
+    w.gstep += S;
+    w.evals += (unsigned long long)__builtin_popcount(w.live) * (64ull * S);
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      if (WaveAny(w.lmin[p] < cmax2)) hit |= 1u << p;
+    hit &= w.live;
+    if (poll) gone = (uint32_t)__ballot(flag != 0u) & w.live & ~hit;
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+#pragma unroll
+      for (int d = 0; d < ND; ++d) w.bcur[k][d] = bnext[k][d];
+    w.step = step_next;
+    cmax2 = Uniform(cm_next);
+    if ((hit | gone) != 0u || w.gstep >= next_due) break;
   }
-  // Reservoir (one candidate per lane) and its valid-lane mask.
+}
+
+template <int ND, int P, int S>
+HD_DEV void ScanSegment(const ScanArgs& args, uint32_t seg, uint32_t rank,
+                        uint32_t lane, ScanWave<ND, P, S>& w,
+                        uint32_t& stg_idx, float& stg_min, uint32_t& n_stg) {
+  constexpr uint32_t kAll = ScanWave<ND, P, S>::kAll;
+  const bool split = args.n_segs > 1;
+  const uint32_t seg_begin = seg * args.seg_steps;
+  const uint32_t seg_len = min(args.seg_steps, args.n_steps - seg_begin);
+  const uint32_t seg_end = seg_begin + seg_len;
+  const uint32_t limit = split ? seg_len : min(args.budget_steps, seg_len);
+  unsigned int* cursor = args.cursors + seg;
+
+  w.live = 0;
+  w.gstep = 0;
+  w.step = seg_begin + (uint32_t)(((uint64_t)rank * 7919u) % seg_len);
+  uint32_t next_due = 0xFFFFFFFFu;
+  // Reservoir: one candidate per lane, claimed 64 positions at a time.
   float rp[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) rp[d] = 0.f;
-  uint32_t ridx = 0;
+  uint32_t ridx = 0, rpos = 0;
   unsigned long long rmask = 0;
   bool drained = false;
-  uint32_t gstep = 0;
-  unsigned long long evals = 0;
-  // Escalations staged in lanes (lane k holds the k-th pending one) and
-  // appended 64 at a time, so the global counter sees one atomic per 64.
-  uint32_t stg_idx = 0;
-  float stg_min = 0.f;
-  uint32_t n_stg = 0;
-  auto flush = [&]() {
+
+  auto flush = [&]() __attribute__((always_inline)) {
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(args.esc_count, n_stg);
     base = Uniform(base);
@@ -174,252 +262,170 @@ __global__ __launch_bounds__(kBlock) void EbSlotKernel(SlotArgs args) {
     n_stg = 0;
   };
 
-  float cmax2 = LoadCmax2(args.key);
-  float bcur[2][ND];
-  LoadStep<ND>(args.bx, args.by, args.bz, step, lane, bcur[0]);
-  LoadStep<ND>(args.bx, args.by, args.bz, NextStep(step, n_steps), lane,
-               bcur[1]);
-
-  while (true) {
-    // Refill.
+  uint32_t cmax2 = __float_as_uint(LoadCmax2(args.key));
+  {
+    uint32_t st = w.step;
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-      if (!(live & (1u << p))) {
-        while (rmask == 0ull && !drained) {
-          uint32_t base = 0;
-          if (lane == 0) base = atomicAdd(args.cursor, (unsigned)kWaveSize);
-          base = Uniform(base);
-          if (base >= args.pos_end) {
-            drained = true;
-            break;
-          }
-          const uint32_t pos = base + lane;
-          bool valid = pos < args.pos_end;
-          if (valid && args.list_min != nullptr)
-            valid = !(args.list_min[pos] < cmax2);
-          if (valid) {
-            ridx = OrderAt(args.order_mode, args.order, args.perm_a, pos);
-            const float* src = args.a + (uint64_t)ridx * ND;
-#pragma unroll
-            for (int d = 0; d < ND; ++d) rp[d] = src[d];
-          }
-          rmask = __ballot(valid);
-        }
-        if (rmask != 0ull) {
-          const uint32_t k = (uint32_t)__builtin_ctzll(rmask);
-          rmask &= rmask - 1ull;
-#pragma unroll
-          for (int d = 0; d < ND; ++d) pa[p][d] = ReadLaneF(rp[d], k);
-          aidx[p] = ReadLaneU(ridx, k);
-          lmin[p] = kFltMax;
-          start[p] = gstep;
-          live |= 1u << p;
-        }
-      }
+    for (int k = 0; k < S; ++k) {
+      LoadStep<ND>(args.bx, args.by, args.bz, st, lane, w.bcur[k]);
+      st = NextInSeg(st, seg_begin, seg_end);
     }
-    if (live == 0) break;
-    const uint32_t n_live = __builtin_popcount(live);
-
-    // Prefetch the next two steps and cmax2, then evaluate the current two.
-    const uint32_t s2 = NextStep(NextStep(step, n_steps), n_steps);
-    const uint32_t s3 = NextStep(s2, n_steps);
-    float bnext[2][ND];
-    LoadStep<ND>(args.bx, args.by, args.bz, s2, lane, bnext[0]);
-    LoadStep<ND>(args.bx, args.by, args.bz, s3, lane, bnext[1]);
-    const uint32_t* cm_ptr = reinterpret_cast<const uint32_t*>(args.key) + 1;
-    const uint32_t cm_next =
-        __hip_atomic_load(cm_ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    EvalStep<ND, P>(pa, live, bcur[0], lmin);
-    EvalStep<ND, P>(pa, live, bcur[1], lmin);
-    step = s2;
-    gstep += 2;
-    evals += (unsigned long long)n_live * 128ull;
-
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-      if (live & (1u << p)) {
-        const uint32_t done = gstep - start[p];
-        if (WaveAny(lmin[p] < cmax2)) {
-          live &= ~(1u << p);  // cannot raise the maximum
-        } else if (done >= n_steps) {
-          // Whole of B scanned: exact minimum.
-          const float m = WaveMin(lmin[p]);
-          if (lane == 0 && m != kFltMax)
-            atomicMax(args.key, PackKey(m, aidx[p]));
-          live &= ~(1u << p);
-        } else if (done >= args.budget_steps) {
-          const float m = WaveMin(lmin[p]);
-          if (lane == n_stg) {
-            stg_idx = aidx[p];
-            stg_min = m;
-          }
-          if (++n_stg == (uint32_t)kWaveSize) flush();
-          live &= ~(1u << p);
-        }
-      }
-    }
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      bcur[0][d] = bnext[0][d];
-      bcur[1][d] = bnext[1][d];
-    }
-    cmax2 = __uint_as_float(Uniform(cm_next));
   }
-  if (n_stg > 0) flush();
-  if (lane == 0) atomicAdd(args.evals, evals);
-}
-
-struct CoopArgs {
-  const float* a;          // AoS A
-  const uint32_t* list;    // A indices, processed in groups of P
-  const float* list_min;   // partial minima (null: none known)
-  uint32_t n_list;
-  const float* bx;
-  const float* by;
-  const float* bz;
-  uint32_t n_steps;        // n_bp / 64
-  uint32_t seg_steps;      // steps per segment
-  uint32_t n_segs;         // segments per group
-  unsigned long long n_items;  // groups * n_segs
-  unsigned long long* claim;   // item cursor
-  uint32_t* broken;        // per list entry: 1 once known not to be a max
-  uint32_t* pmin;          // per list entry: FP32 bits of the running min
-  uint32_t* gdone;         // per group: completed segments
-  unsigned long long* key;
-  unsigned long long* evals;
-};
-
-HD_DEV void MarkBroken(uint32_t* flag) {
-  __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Stages 1, 3 and the tail: work item = (group of P list entries, B
-// segment); items of one group are consecutive, so a group is spread over
-// many waves at once. Entry data is loaded lane-parallel (lane p < P loads
-// entry p) and handed to the uniform slots with readlane.
-template <int ND, int P>
-__global__ __launch_bounds__(kBlock) void EbCoopKernel(CoopArgs args) {
-  const uint32_t lane = LaneId();
-  const uint32_t n_steps = args.n_steps;
-  float pa[P][ND];
-  float lmin[P];
-  unsigned long long evals = 0;
-
   while (true) {
-    unsigned long long t = 0;
-    if (lane == 0) t = atomicAdd(args.claim, 1ull);
-    t = __builtin_amdgcn_readfirstlane((uint32_t)t) |
-        ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(t >> 32))
-         << 32);
-    if (t >= args.n_items) break;
-    const uint32_t g = (uint32_t)(t / args.n_segs);
-    const uint32_t s = (uint32_t)(t % args.n_segs);
-
-    float cmax2 = LoadCmax2(args.key);
-    // Lane-parallel entry load.
-    const uint32_t e_lane = g * P + lane;
-    bool ok = lane < (uint32_t)P && e_lane < args.n_list;
-    float ep[ND];
+    // Refill free slots from the reservoir.
+    if (w.live != kAll) {
 #pragma unroll
-    for (int d = 0; d < ND; ++d) ep[d] = 0.f;
-    if (ok) {
-      const uint32_t flag = __hip_atomic_load(
-          &args.broken[e_lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (flag != 0u) {
-        ok = false;
-      } else if (args.list_min != nullptr && args.list_min[e_lane] < cmax2) {
-        MarkBroken(&args.broken[e_lane]);
-        ok = false;
-      } else {
-        const float* src = args.a + (uint64_t)args.list[e_lane] * ND;
+      for (int p = 0; p < P; ++p) {
+        if (!(w.live & (1u << p))) {
+          while (rmask == 0ull && !drained) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(cursor, args.claim);
+            base = Uniform(base);
+            if (base >= args.pos_end) {
+              drained = true;
+              break;
+            }
+            const uint32_t pos = base + lane;
+            bool valid = lane < args.claim && pos < args.pos_end;
+            if (valid && args.list_min != nullptr)
+              valid = !(__float_as_uint(args.list_min[pos]) < cmax2);
+            if (valid && split)
+              valid = __hip_atomic_load(&args.broken[pos], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT) == 0u;
+            if (valid) {
+              ridx = OrderAt(args.order_mode, args.order, args.perm_a, pos);
+              rpos = pos;
+              const float* src = args.a + (uint64_t)ridx * ND;
 #pragma unroll
-        for (int d = 0; d < ND; ++d) ep[d] = src[d];
+              for (int d = 0; d < ND; ++d) rp[d] = src[d];
+            }
+            rmask = __ballot(valid);
+          }
+          if (rmask != 0ull) {
+            const uint32_t k = (uint32_t)__builtin_ctzll(rmask);
+            rmask &= rmask - 1ull;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) w.pa[p][d] = ReadLaneF(rp[d], k);
+            w.aidx[p] = ReadLaneU(ridx, k);
+            w.lpos[p] = ReadLaneU(rpos, k);
+            if (lane == (uint32_t)p) w.lane_pos = w.lpos[p];
+            w.lmin[p] = kFltMaxBits;
+            w.start[p] = w.gstep;
+            w.live |= 1u << p;
+            next_due = min(next_due, w.gstep + limit);
+          }
+        }
       }
     }
-    uint32_t live = (uint32_t)__ballot(ok);
-    if (live == 0) {
-      // Every entry of the group is decided: skip its remaining segments.
-      if (lane == 0)
-        atomicMax(args.claim, (unsigned long long)(g + 1) * args.n_segs);
-      continue;
-    }
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-      lmin[p] = kFltMax;
-#pragma unroll
-      for (int d = 0; d < ND; ++d) pa[p][d] = ReadLaneF(ep[d], p);
-    }
+    if (w.live == 0) break;
 
-    const uint32_t st_begin = s * args.seg_steps;
-    const uint32_t st_end = min(st_begin + args.seg_steps, n_steps);
-    uint32_t since_flags = 0;
-    float bcur[ND];
-    LoadStep<ND>(args.bx, args.by, args.bz, st_begin, lane, bcur);
-    const uint32_t* cm_ptr = reinterpret_cast<const uint32_t*>(args.key) + 1;
-    for (uint32_t st = st_begin; st < st_end; ++st) {
-      // Prefetch the next step (clamped inside the segment) and cmax2.
-      float bnext[ND];
-      LoadStep<ND>(args.bx, args.by, args.bz, st + 1 < st_end ? st + 1 : st,
-                   lane, bnext);
-      const uint32_t cm_next = __hip_atomic_load(
-          cm_ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      EvalStep<ND, P>(pa, live, bcur, lmin);
-      evals += (unsigned long long)__builtin_popcount(live) * 64ull;
-      if (((st - st_begin) & 1u) == 1u || st + 1 == st_end) {
-        // Break checks every two steps.
-        const bool refresh = (++since_flags & 7u) == 0u;
-        bool flagged = false;
-        if (refresh && lane < (uint32_t)P && (live & (1u << lane)))
-          flagged = __hip_atomic_load(&args.broken[g * P + lane],
-                                      __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT) != 0u;
-        const uint32_t flagged_mask = (uint32_t)__ballot(flagged);
+    // Iterate until a slot breaks, is due, or is flagged by another segment.
+    // The loop is instantiated twice: every slot live (no per-slot
+    // branches) and some slots free (the cursor has drained).
+    uint32_t hit = 0, gone = 0;
+    if (w.live == kAll)
+      ScanIterate<ND, P, S, true>(args, w, lane, seg_begin, seg_end, split,
+                                  next_due, cmax2, hit, gone);
+    else
+      ScanIterate<ND, P, S, false>(args, w, lane, seg_begin, seg_end, split,
+                                   next_due, cmax2, hit, gone);
+
+    w.live &= ~(hit | gone);
+    if (w.gstep >= next_due) {
+      uint32_t nd = 0xFFFFFFFFu;
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-          if (live & (1u << p)) {
-            if (WaveAny(lmin[p] < cmax2)) {
-              if (lane == 0) MarkBroken(&args.broken[g * P + p]);
-              live &= ~(1u << p);
-            } else if (flagged_mask & (1u << p)) {
-              live &= ~(1u << p);  // another segment broke it
+      for (int p = 0; p < P; ++p) {
+        if (!(w.live & (1u << p))) continue;
+        const uint32_t done = w.gstep - w.start[p];
+        if (done < limit) {
+          nd = min(nd, w.start[p] + limit);
+          continue;
+        }
+        const uint32_t mb = WaveMinU32(w.lmin[p]);
+        w.live &= ~(1u << p);
+        if (done >= seg_len) {
+          if (!split) {
+            // Whole of B scanned: exact minimum.
+            if (lane == 0 && mb != kFltMaxBits)
+              atomicMax(args.key, PackKey(__uint_as_float(mb), w.aidx[p]));
+          } else if (lane == 0) {
+            // Segment minimum, then the completion count. The count's
+            // increment depends on the value the atomic min returned, so it
+            // is issued only after the min has been performed at the device
+            // coherence point; the wave whose increment completes the count
+            // then reads the minimum over every segment there. Segments that
+            // broke or were dropped never count, so a full count means an
+            // exact minimum. (Fences would also order this, but an acquire
+            // invalidates the L2 that holds B.)
+            const uint32_t e = w.lpos[p];
+            const uint32_t was = atomicMin(&args.pmin[e], mb);
+            const uint32_t prev = __hip_atomic_fetch_add(
+                &args.gdone[e], 1u + (was >> 31), __ATOMIC_RELAXED,
+                __HIP_MEMORY_SCOPE_AGENT);
+            if (prev + 1u == args.n_segs) {
+              const uint32_t bits = __hip_atomic_load(
+                  &args.pmin[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              if (bits != kFltMaxBits)
+                atomicMax(args.key, PackKey(__uint_as_float(bits), w.aidx[p]));
             }
           }
+        } else {
+          if (lane == n_stg) {
+            stg_idx = w.aidx[p];
+            stg_min = __uint_as_float(mb);
+          }
+          if (++n_stg == (uint32_t)kWaveSize) flush();
         }
-        if (live == 0) break;
       }
-#pragma unroll
-      for (int d = 0; d < ND; ++d) bcur[d] = bnext[d];
-      cmax2 = __uint_as_float(Uniform(cm_next));
+      next_due = nd;
     }
-
-    // Publish segment minima, then count the segment as done.
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-      if (live & (1u << p)) {
-        const float m = WaveMin(lmin[p]);
-        if (lane == 0) atomicMin(&args.pmin[g * P + p], __float_as_uint(m));
-      }
+    if (split && hit != 0u && lane < (uint32_t)P && ((hit >> lane) & 1u)) {
+      // Tell the other segments' waves this entry is decided.
+      __hip_atomic_store(&args.broken[w.lane_pos], 1u, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     }
-    uint32_t prev = 0;
-    if (lane == 0)
-      prev = __hip_atomic_fetch_add(&args.gdone[g], 1u, __ATOMIC_ACQ_REL,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-    prev = Uniform(prev);
-    if (prev + 1 == args.n_segs && lane < (uint32_t)P) {
-      // Last segment of the group: every segment has published; entries
-      // never marked broken were scanned over all of B.
-      const uint32_t e = g * P + lane;
-      if (e < args.n_list &&
-          __hip_atomic_load(&args.broken[e], __ATOMIC_ACQUIRE,
-                            __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-        const uint32_t bits = __hip_atomic_load(
-            &args.pmin[e], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        if (bits != kFltMaxBits)
-          atomicMax(args.key, PackKey(__uint_as_float(bits), args.list[e]));
-      }
+    if ((hit | gone) != 0u) {
+      // Recompute the earliest due step over the remaining slots.
+      uint32_t nd = 0xFFFFFFFFu;
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+        if (w.live & (1u << p)) nd = min(nd, w.start[p] + limit);
+      next_due = nd;
     }
   }
-  if (lane == 0) atomicAdd(args.evals, evals);
+  if (n_stg > 0) flush();
+}
+
+template <int ND, int P, int S>
+__global__ __launch_bounds__(kBlock) void EbScanKernel(ScanArgs args) {
+  const uint32_t lane = LaneId();
+  const uint32_t wave_id = Uniform((blockIdx.x * kBlock + threadIdx.x) >> 6);
+  const uint32_t n_waves = gridDim.x * (kBlock / kWaveSize);
+  ScanWave<ND, P, S> w;
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    w.aidx[p] = 0;
+    w.lpos[p] = 0;
+    w.start[p] = 0;
+    w.lmin[p] = kFltMaxBits;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) w.pa[p][d] = 0.f;
+  }
+  w.evals = 0;
+  w.lane_pos = 0;
+  // Escalations staged in lanes (lane k holds the k-th pending one) and
+  // appended 64 at a time.
+  uint32_t stg_idx = 0, n_stg = 0;
+  float stg_min = 0.f;
+  // Wave k of the sweep takes segment k % n_segs; with fewer waves than
+  // segments a wave takes segments wave_id, wave_id + n_waves, ...
+  const uint32_t sweep = max(n_waves, args.n_segs);
+  for (uint32_t k = wave_id; k < sweep; k += n_waves) {
+    const uint32_t seg = k % args.n_segs;
+    ScanSegment<ND, P, S>(args, seg, k / args.n_segs, lane, w, stg_idx,
+                          stg_min, n_stg);
+  }
+  if (lane == 0) atomicAdd(args.evals, w.evals);
 }
 
 // Seed list: the first n order positions as A indices.
@@ -430,65 +436,93 @@ __global__ void SeedListKernel(const uint32_t* order, Feistel perm,
     out[i] = OrderAt(order_mode, order, perm, i);
 }
 
-constexpr int kP = 8;  // slots per wave / points per cooperative group
+constexpr int kP = 8;  // slots per wave
+constexpr int kS = 4;  // B steps per iteration
 
-// Cooperative kernel over `list` (groups of kP entries). Each group's B scan
-// is split into n_segs segments processed by different waves: as many as
-// keep every wave busy, but at least 64 steps each, so a group that breaks
-// early wastes at most n_segs short partial scans.
+// Blocks of the scan kernel that are resident at once (all CUs at the
+// kernel's occupancy), so that no block of a persistent launch waits for
+// another to exit.
 template <int ND>
-int LaunchCoop(hd_context* ctx, const float* a, const uint32_t* list,
-               const float* list_min, uint32_t n_list, const float* bx,
-               const float* by, const float* bz, uint32_t n_steps,
-               unsigned long long* key, unsigned long long* evals,
-               uint32_t min_segs = 1) {
+int ResidentBlocks(hd_context* ctx) {
+  static int per_cu[4] = {0, 0, 0, 0};
+  if (per_cu[ND] == 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, EbScanKernel<ND, kP, kS>, kBlock, 0) != hipSuccess || n < 1)
+      n = 1;
+    per_cu[ND] = n;
+  }
+  return per_cu[ND] * ctx->cu_count;
+}
+
+template <int ND>
+int LaunchScan(hd_context* ctx, ScanArgs& args, uint32_t pos_begin,
+               uint32_t n_entries) {
+  HD_TRY(ContextArray(ctx, "eb_cursors", args.n_segs, &args.cursors));
+  HD_TRY(FillU32(ctx, args.cursors, pos_begin, args.n_segs));
+  const unsigned long long items =
+      ((unsigned long long)n_entries + kP - 1) / kP * args.n_segs;
+  int blocks = ResidentBlocks<ND>(ctx);
+  if ((unsigned long long)blocks * 4 > items)
+    blocks = (int)std::max<unsigned long long>(1, (items + 3) / 4);
+  // Small lists: claim fewer positions at a time so that every wave of a
+  // segment gets a share.
+  const uint32_t n_waves = (uint32_t)blocks * (kBlock / kWaveSize);
+  const uint32_t per_seg = std::max(1u, n_waves / args.n_segs);
+  args.claim = std::min<uint32_t>(
+      kWaveSize, std::max<uint32_t>(1u, (n_entries + per_seg - 1) / per_seg));
+  EbScanKernel<ND, kP, kS><<<blocks, kBlock, 0, ctx->stream>>>(args);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
+
+// Exact minima of the entries of `list` with B split into segments scanned
+// by different waves: as many as keep every slot busy, at least min_segs,
+// at least 64 steps each.
+template <int ND>
+int LaunchSplit(hd_context* ctx, const float* a, const uint32_t* list,
+                const float* list_min, uint32_t n_list, const float* bx,
+                const float* by, const float* bz, uint32_t n_steps,
+                unsigned long long* key, unsigned long long* evals,
+                uint32_t min_segs = 1) {
   if (n_list == 0) return HD_OK;
-  const uint32_t n_groups = (n_list + kP - 1) / kP;
-  const uint32_t total_waves =
-      (uint32_t)PersistentBlocks(ctx, kBlock, 8) * (kBlock / kWaveSize);
-  uint32_t n_segs = std::max(min_segs, total_waves / n_groups);
+  const uint32_t total_slots =
+      (uint32_t)ResidentBlocks<ND>(ctx) * (kBlock / kWaveSize) * kP;
+  const uint32_t want = (4u * total_slots + n_list - 1) / n_list;
+  uint32_t n_segs = std::max(min_segs, want);
   n_segs = std::min(n_segs, std::max(1u, n_steps / 64u));
   const uint32_t seg_steps = (n_steps + n_segs - 1) / n_segs;
   n_segs = (n_steps + seg_steps - 1) / seg_steps;
-  unsigned long long* claim = nullptr;
-  uint32_t *broken = nullptr, *pmin = nullptr, *gdone = nullptr;
-  HD_TRY(ContextArray(ctx, "coop_claim", 1, &claim));
-  HD_TRY(ContextArray(ctx, "coop_broken", n_list, &broken));
-  HD_TRY(ContextArray(ctx, "coop_pmin", n_list, &pmin));
-  HD_TRY(ContextArray(ctx, "coop_gdone", n_groups, &gdone));
-  HD_TRY(FillU64(ctx, claim, 0ull, 1));
-  HD_TRY(FillU32(ctx, broken, 0u, n_list));
-  HD_TRY(FillU32(ctx, pmin, kFltMaxBits, n_list));
-  HD_TRY(FillU32(ctx, gdone, 0u, n_groups));
-  CoopArgs args;
+  ScanArgs args = {};
   args.a = a;
-  args.list = list;
+  args.order = list;
   args.list_min = list_min;
-  args.n_list = n_list;
+  args.perm_a = MakeFeistel(1, 0, 0);
+  args.order_mode = kOrderArray;
+  args.pos_end = n_list;
   args.bx = bx;
   args.by = by;
   args.bz = bz;
   args.n_steps = n_steps;
   args.seg_steps = seg_steps;
   args.n_segs = n_segs;
-  args.n_items = (unsigned long long)n_groups * n_segs;
-  args.claim = claim;
-  args.broken = broken;
-  args.pmin = pmin;
-  args.gdone = gdone;
+  args.budget_steps = 0xFFFFFFFFu;
   args.key = key;
   args.evals = evals;
-  int blocks = PersistentBlocks(ctx, kBlock, 8);
-  const unsigned long long waves_needed = args.n_items;
-  if ((unsigned long long)blocks * 4 > waves_needed)
-    blocks = (int)std::max<unsigned long long>(1, (waves_needed + 3) / 4);
-  EbCoopKernel<ND, kP><<<blocks, kBlock, 0, ctx->stream>>>(args);
-  HD_HIP_TRY(hipGetLastError());
-  return HD_OK;
+  if (n_segs > 1) {
+    HD_TRY(ContextArray(ctx, "eb_broken", n_list, &args.broken));
+    HD_TRY(ContextArray(ctx, "eb_pmin", n_list, &args.pmin));
+    HD_TRY(ContextArray(ctx, "eb_gdone", n_list, &args.gdone));
+    HD_TRY(FillU32(ctx, args.broken, 0u, n_list));
+    HD_TRY(FillU32(ctx, args.pmin, kFltMaxBits, n_list));
+    HD_TRY(FillU32(ctx, args.gdone, 0u, n_list));
+  }
+  return LaunchScan<ND>(ctx, args, 0u, n_list);
 }
 
-// Slot kernel over order positions [pos_begin, pos_end); escalations are
-// appended to (esc_idx, esc_min) and counted in *esc_count.
+// Whole-B scans over order positions [pos_begin, pos_end) with a step
+// budget; escalations are appended to (esc_idx, esc_min) and counted in
+// counters[1].
 template <int ND>
 int LaunchSlots(hd_context* ctx, const float* a, int order_mode,
                 const uint32_t* order, const float* list_min,
@@ -496,36 +530,29 @@ int LaunchSlots(hd_context* ctx, const float* a, int order_mode,
                 const float* bx, const float* by, const float* bz,
                 uint32_t n_steps, uint32_t budget_steps,
                 unsigned long long* key, unsigned long long* evals,
-                unsigned int* counters /* cursor, esc_count */,
-                uint32_t* esc_idx, float* esc_min) {
+                unsigned int* counters, uint32_t* esc_idx, float* esc_min) {
   if (pos_begin >= pos_end) return HD_OK;
-  HD_TRY(FillU32(ctx, counters, pos_begin, 1));
   HD_TRY(FillU32(ctx, counters + 1, 0u, 1));
-  SlotArgs sa;
-  sa.a = a;
-  sa.order = order;
-  sa.list_min = list_min;
-  sa.perm_a = perm_a;
-  sa.order_mode = order_mode;
-  sa.pos_end = pos_end;
-  sa.bx = bx;
-  sa.by = by;
-  sa.bz = bz;
-  sa.n_steps = n_steps;
-  sa.budget_steps = budget_steps;
-  sa.key = key;
-  sa.cursor = counters;
-  sa.esc_idx = esc_idx;
-  sa.esc_min = esc_min;
-  sa.esc_count = counters + 1;
-  sa.evals = evals;
-  const uint32_t waves_needed = (pos_end - pos_begin + kP - 1) / kP;
-  int blocks = PersistentBlocks(ctx, kBlock, 8);
-  if ((uint32_t)blocks * 4u > waves_needed)
-    blocks = (int)std::max(1u, (waves_needed + 3u) / 4u);
-  EbSlotKernel<ND, kP><<<blocks, kBlock, 0, ctx->stream>>>(sa);
-  HD_HIP_TRY(hipGetLastError());
-  return HD_OK;
+  ScanArgs args = {};
+  args.a = a;
+  args.order = order;
+  args.list_min = list_min;
+  args.perm_a = perm_a;
+  args.order_mode = order_mode;
+  args.pos_end = pos_end;
+  args.bx = bx;
+  args.by = by;
+  args.bz = bz;
+  args.n_steps = n_steps;
+  args.seg_steps = n_steps;
+  args.n_segs = 1;
+  args.budget_steps = budget_steps;
+  args.key = key;
+  args.esc_idx = esc_idx;
+  args.esc_min = esc_min;
+  args.esc_count = counters + 1;
+  args.evals = evals;
+  return LaunchScan<ND>(ctx, args, pos_begin, pos_end - pos_begin);
 }
 
 int ReadCount(hd_context* ctx, const unsigned int* d_count, uint32_t* out) {
@@ -603,7 +630,7 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
     const uint32_t k_top = std::min<uint32_t>(n_esc, 256u);
     rs->top_points += k_top;
     const int st_top = StageBegin(ctx, "eb_top");
-    HD_TRY((LaunchCoop<ND>(ctx, a, idx_sorted, min_sorted, k_top, t.bx, t.by,
+    HD_TRY((LaunchSplit<ND>(ctx, a, idx_sorted, min_sorted, k_top, t.bx, t.by,
                            t.bz, t.n_steps, key, ev + 3)));
     StageEnd(ctx, st_top);
     // The sorted escalations become the next round's list; the next round
@@ -625,7 +652,7 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
     // each group split over at least 16 waves.
     rs->tail_points += end - begin;
     const int st_tail = StageBegin(ctx, "eb_tail");
-    HD_TRY((LaunchCoop<ND>(ctx, a, list + begin,
+    HD_TRY((LaunchSplit<ND>(ctx, a, list + begin,
                            list_min ? list_min + begin : nullptr, end - begin,
                            t.bx, t.by, t.bz, t.n_steps, key, ev + 4, 16u)));
     StageEnd(ctx, st_tail);
@@ -668,7 +695,7 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
   SeedListKernel<<<std::max(1u, (s0 + 255u) / 256u), 256, 0, ctx->stream>>>(
       order, perm_a, order_mode, s0, seed_list);
   HD_HIP_TRY(hipGetLastError());
-  HD_TRY((LaunchCoop<ND>(ctx, in.a, seed_list, nullptr, s0, t.bx, t.by, t.bz,
+  HD_TRY((LaunchSplit<ND>(ctx, in.a, seed_list, nullptr, s0, t.bx, t.by, t.bz,
                          t.n_steps, key, ev + 0)));
   StageEnd(ctx, st_seed);
 

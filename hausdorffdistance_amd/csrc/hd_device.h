@@ -38,6 +38,12 @@ HD_DEV float D2(const float* a, float bx, float by, float bz) {
   }
 }
 
+// d2 is +0 or positive (never -0; never NaN for finite inputs), so its FP32
+// bit pattern orders like the value. Running minima are kept as bits and
+// reduced with an unsigned min: one instruction, where fminf also has to
+// quiet signalling NaNs first.
+HD_DEV uint32_t MinBits(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
 // 64-bit key whose unsigned order is (d2 ascending, index descending); the
 // maximum key is the largest d2 at the smallest index. d2 >= +0, so the FP32
 // bit pattern orders like the value.
