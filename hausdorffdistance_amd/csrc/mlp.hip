@@ -4,11 +4,16 @@
 // counterpart in the reference, whose selector uses gradient-boosted trees
 // exported as C (src/models/tree_*.h) for parameters, not for order.
 //
-// Network: d_in -> 64 -> 64 -> 1, ReLU, BF16 weights (an FP32 master copy is
-// what Adam updates), FP32 accumulation. Input: coordinates scaled by the
-// training set's MBR; target: standardised log NN distance. Training labels
-// are exact NN distances of a subsample (grid query). The exact search still
-// decides every value: the order changes only how soon cmax2 rises.
+// Network: d_in -> 64 -> 64 -> 1, sine first layer (h1 = sin(W1 x + b1)),
+// ReLU second layer, linear output; BF16 weights (an FP32 master copy is
+// what Adam updates), FP32 accumulation. The sine layer is a learned
+// Fourier basis (as in SIREN, Sitzmann et al., NeurIPS 2020): NN distances
+// of clustered data vary on the scale of a cluster, far finer than what 64
+// ReLU hyperplanes over the whole extent resolve. Input: coordinates scaled
+// by the training set's MBR; target: standardised log NN distance. Training
+// labels are exact NN distances of a subsample (grid query). The exact
+// search still decides every value: the order changes only how soon cmax2
+// rises.
 //
 // Kernels: forward+backward with one lane per sample (weights in LDS as
 // FP32 converted from BF16), a per-parameter kernel that reduces the
@@ -105,17 +110,22 @@ HD_DEV void LoadWeights(const uint16_t* __restrict__ wbf, float* s_w) {
   __syncthreads();
 }
 
-// First layer into registers (fully unrolled so h1 stays in VGPRs).
+// First layer pre-activation of unit j.
+template <int ND>
+HD_DEV float Layer1Pre(const float* s_w, const float* x, int j) {
+  const Layout L = MakeLayout(ND);
+  float acc = s_w[L.b1 + j];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) acc = fmaf(s_w[L.w1 + j * ND + k], x[k], acc);
+  return acc;
+}
+
+// First layer into registers (fully unrolled so h1 stays in VGPRs). The
+// hardware sine (|argument| stays within a few hundred radians here).
 template <int ND>
 HD_DEV void Layer1(const float* s_w, const float* x, float (&h1)[kH]) {
-  const Layout L = MakeLayout(ND);
 #pragma unroll
-  for (int j = 0; j < kH; ++j) {
-    float acc = s_w[L.b1 + j];
-#pragma unroll
-    for (int k = 0; k < ND; ++k) acc = fmaf(s_w[L.w1 + j * ND + k], x[k], acc);
-    h1[j] = fmaxf(acc, 0.f);
-  }
+  for (int j = 0; j < kH; ++j) h1[j] = __sinf(Layer1Pre<ND>(s_w, x, j));
 }
 
 // Hidden unit j of the second layer (pre-activation).
@@ -202,7 +212,7 @@ __global__ __launch_bounds__(256) void MlpForwardBackwardKernel(TrainArgs a) {
 #pragma unroll
     for (int k = 0; k < kH; ++k) {
       a.h1[(uint64_t)s * kH + k] = h1[k];
-      a.g1[(uint64_t)s * kH + k] = h1[k] > 0.f ? g1[k] : 0.f;
+      a.g1[(uint64_t)s * kH + k] = g1[k] * __cosf(Layer1Pre<ND>(s_w, x, k));
     }
   }
 #pragma unroll
@@ -358,10 +368,12 @@ void FreeModel(hd_mlp* m) {
 
 // Initialisation from SplitMix64 (+ Box-Muller). First layer: random
 // hyperplanes through the unit box of the normalised inputs, with slopes of
-// order kW1Scale, so the 64 ReLU kinks tile the data domain finely enough to
-// resolve cluster-scale structure (targets vary over ~1/50 of the extent);
-// deeper layers He-normal; biases of layers 2 and 3 zero.
-constexpr double kW1Scale = 24.0;
+// kOmega (first-layer weights uniform in [-kOmega/2, kOmega/2], biases in
+// [-kOmega, kOmega]): sine features of up to ~kOmega/(2 pi) periods over the
+// unit square, fine enough for cluster-scale structure (targets vary over
+// ~1/50 of the extent); deeper layers He-normal; biases of layers 2 and 3
+// zero.
+constexpr double kOmega = 90.0;
 void InitParams(int nd, uint32_t seed, std::vector<float>* p) {
   const Layout L = HostLayout(nd);
   p->assign(L.n, 0.f);
@@ -375,14 +387,9 @@ void InitParams(int nd, uint32_t seed, std::vector<float>* p) {
     return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
   };
   for (int j = 0; j < kH; ++j) {
-    double dot = 0;
-    for (int i = 0; i < nd; ++i) {
-      const double w = normal() * kW1Scale / std::sqrt((double)nd);
-      const double c = uniform();  // the hyperplane passes through c
-      (*p)[L.w1 + j * nd + i] = (float)w;
-      dot += w * c;
-    }
-    (*p)[L.b1 + j] = (float)(-dot);
+    for (int i = 0; i < nd; ++i)
+      (*p)[L.w1 + j * nd + i] = (float)((uniform() - 0.5) * kOmega);
+    (*p)[L.b1 + j] = (float)((2.0 * uniform() - 1.0) * kOmega);
   }
   for (int i = 0; i < kH * kH; ++i) (*p)[L.w2 + i] = (float)(normal() * std::sqrt(2.0 / kH));
   for (int i = 0; i < kH; ++i) (*p)[L.w3 + i] = (float)(normal() * std::sqrt(1.0 / kH));

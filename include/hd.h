@@ -180,7 +180,7 @@ int hd_batch_directed_device(hd_context* ctx, int n_dims, const float* d_a,
 
 /* ---- learned visit order (no reference counterpart: the reference's
  * selector uses gradient-boosted trees, src/models/tree_*.h) ----
- * MLP n_dims -> 64 -> 64 -> 1, ReLU, BF16 weights with an FP32 master copy,
+ * MLP n_dims -> 64 -> 64 -> 1 (sine, ReLU, linear), BF16 weights with an FP32 master copy,
  * FP32 accumulation. Input: coordinates normalised by the training set's
  * MBR. Output: predicted log of the NN distance. */
 typedef struct hd_mlp_train_config {

@@ -400,7 +400,8 @@ void orc_mbr(const float* p, uint64_t n, int n_dims, float* out) {
 }
 
 /* FP64 forward/backward of the ordering MLP (no reference counterpart: the
- * reference's selector uses gradient-boosted trees, src/models/tree_*.h).
+ * reference's selector uses gradient-boosted trees, src/models/tree_*.h):
+ * h1 = sin(W1 x + b1), h2 = relu(W2 h1 + b2), f = w3 . h2 + b3.
  * Loss = mean over the batch of (f(x) - y)^2. */
 double orc_mlp_loss_grad(const float* params, int d_in, int hidden,
                          const float* x, const float* y, int batch,
@@ -420,6 +421,7 @@ double orc_mlp_loss_grad(const float* params, int d_in, int hidden,
   double* gw3 = gb2 + H;
   double* gb3 = gw3 + H;
   memset(grad, 0, sizeof(double) * n_params);
+  double* z1 = (double*)malloc(sizeof(double) * H);
   double* h1 = (double*)malloc(sizeof(double) * H);
   double* h2 = (double*)malloc(sizeof(double) * H);
   double* g2 = (double*)malloc(sizeof(double) * H);
@@ -430,7 +432,8 @@ double orc_mlp_loss_grad(const float* params, int d_in, int hidden,
     for (int j = 0; j < H; ++j) {
       double acc = b1[j];
       for (int k = 0; k < d_in; ++k) acc += (double)W1[j * d_in + k] * xs[k];
-      h1[j] = acc > 0 ? acc : 0;
+      z1[j] = acc;
+      h1[j] = sin(acc);
     }
     for (int j = 0; j < H; ++j) {
       double acc = b2[j];
@@ -454,13 +457,14 @@ double orc_mlp_loss_grad(const float* params, int d_in, int hidden,
     for (int k = 0; k < H; ++k) {
       double acc = 0;
       for (int j = 0; j < H; ++j) acc += g2[j] * W2[j * H + k];
-      g1[k] = h1[k] > 0 ? acc : 0;
+      g1[k] = acc * cos(z1[k]);
     }
     for (int j = 0; j < H; ++j) {
       gb1[j] += g1[j];
       for (int k = 0; k < d_in; ++k) gW1[j * d_in + k] += g1[j] * xs[k];
     }
   }
+  free(z1);
   free(h1);
   free(h2);
   free(g2);
