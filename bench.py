@@ -8,9 +8,10 @@ order and the learned order are both timed, `value` is the learned order's
 A-points/s when the learned path is available, else the plain one's.
 
 Prints ONE JSON line. The roofline object is for the dominant kernel (the
-early-break slot kernel), its duration measured with HIP events on the
-library's stream; the CPU baseline is the CPU oracle (a C restatement of the
-reference's CPU early break) on a bounded sample of the same workload.
+early-break kernel with the most device time: the matrix-core filter of the
+budgeted rounds, or the exact scan), its duration measured with HIP events on
+the library's stream; the CPU baseline is the CPU oracle (a C restatement of
+the reference's CPU early break) on a bounded sample of the same workload.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
@@ -33,10 +34,18 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # VALU lane-op ceiling: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz.
 VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
 LANE_OPS_PER_EVAL = {2: 6, 3: 9}
-# The early-break scan kernel and the pipeline stages that launch it.
-SCAN_KERNEL = "EbScanKernel"
-SCAN_KERNEL_FULL = "EbScanKernel<2,8,4>"
-SCAN_STAGES = ("eb_seed", "eb_slot", "eb_slot_more", "eb_top", "eb_tail")
+# Dense FP32 matrix-core peak (MI355X).
+MFMA_F32_TFLOPS = 157.3
+# The early-break kernels and the pipeline stages that launch them: the
+# exact scan (seed, unfiltered rounds, top completions, tail) and the
+# matrix-core filter of the budgeted rounds (16x16x4 FP32 MFMA: 4 MACs =
+# 8 FLOP per A/B pair).
+EB_KERNELS = {
+    "EbScanKernel": {"full": "EbScanKernel<2,8,4>",
+                     "stages": ("eb_seed", "eb_slot", "eb_slot_more", "eb_top", "eb_tail")},
+    "EbMfmaKernel": {"full": "EbMfmaKernel<2>", "stages": ("eb_mfma", "eb_mfma_more")},
+}
+MFMA_FLOP_PER_PAIR = 8
 
 
 def log(msg: str) -> None:
@@ -71,6 +80,54 @@ def time_steps(hd, ctx, variant, da, db, steps, warmup, learned, model=None):
     stages = stage_times(lib, ctx, reset=True)
     lib.hd_context_set_profiling(ctx.handle, 0)
     return (t1 - t0) * 1e3 / steps, results, stages
+
+
+def eb_roofline(stages: dict, steps: int) -> dict:
+    """Roofline of the dominant early-break kernel over the timed plain-order
+    steps: the kernel whose stages take the most device time (HIP events on
+    the library's stream around each stage), its average launch duration,
+    and its algorithmic work per launch."""
+    per = {}
+    for name, k in EB_KERNELS.items():
+        ms = sum(stages.get(st, {}).get("ms", 0.0) for st in k["stages"])
+        calls = sum(stages.get(st, {}).get("calls", 0) for st in k["stages"])
+        evals = sum(stages.get(st, {}).get("evals", 0) for st in k["stages"])
+        points = sum(stages.get(st, {}).get("points", 0) for st in k["stages"])
+        per[name] = (ms, calls, evals, points)
+    name = max(per, key=lambda n: per[n][0])
+    ms, calls, evals, points = per[name]
+    launches = max(1, calls)
+    t_kernel = ms / launches / 1e3
+    traffic = None
+    traffic_file = ROOT / "profiles" / f"traffic_{name}_gmm_10m_2d.json"
+    if traffic_file.exists():
+        tr = json.loads(traffic_file.read_text())
+        if tr.get("kernel") == EB_KERNELS[name]["full"]:
+            traffic = tr.get("hbm_bytes_per_launch")
+    other = {n: {"ms_per_step": v[0] / max(1, steps), "launches_per_step": v[1] / max(1, steps),
+                 "evals_per_step": v[2] / max(1, steps)} for n, v in per.items()}
+    if name == "EbMfmaKernel":
+        flop = MFMA_FLOP_PER_PAIR * evals / launches
+        achieved = flop / t_kernel / 1e12 if t_kernel > 0 else None
+        return {
+            "bound": "mfma", "kernel": EB_KERNELS[name]["full"], "achieved": achieved,
+            "peak": MFMA_F32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / MFMA_F32_TFLOPS if achieved else None,
+            "traffic": traffic, "algorithmic_flop": flop, "kernel_ms": t_kernel * 1e3,
+            "launches_per_step": calls / max(1, steps), "kernels": other,
+        }
+    bytes_alg = 8.0 * (points + evals) / launches
+    achieved = bytes_alg / t_kernel / 1e9 if t_kernel > 0 else None
+    return {
+        "bound": "hbm", "kernel": EB_KERNELS[name]["full"], "achieved": achieved,
+        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS if achieved else None,
+        "traffic": traffic, "algorithmic_bytes": bytes_alg, "kernel_ms": t_kernel * 1e3,
+        "launches_per_step": calls / max(1, steps),
+        "valu_frac": (evals / launches * LANE_OPS_PER_EVAL[2] / t_kernel / VALU_LANE_OPS)
+        if t_kernel > 0 else None,
+        "kernels": other,
+    }
 
 
 def cpu_baseline(a: np.ndarray, b: np.ndarray, target_s: float = 12.0) -> dict:
@@ -327,43 +384,7 @@ def main() -> None:
     n_a = a.shape[0]
     value = n_a / (ms / 1e3)
 
-    # Roofline of the dominant kernel: every EbScanKernel launch of the
-    # plain-order steps (seed, the three budgeted rounds, the top completions
-    # and the tail), HIP events on the library's stream around each stage.
-    scan_ms = 0.0
-    scan_launches = 0
-    scan_evals = 0
-    scan_points = 0
-    for name in SCAN_STAGES:
-        st = st_plain.get(name, {})
-        scan_ms += st.get("ms", 0.0)
-        scan_launches += st.get("calls", 0)
-        scan_evals += st.get("evals", 0)
-        scan_points += st.get("points", 0)
-    launches = max(1, scan_launches)
-    t_kernel = scan_ms / launches / 1e3
-    bytes_alg = 8.0 * (scan_points + scan_evals) / launches
-    achieved = bytes_alg / t_kernel / 1e9 if t_kernel > 0 else None
-    traffic = None
-    traffic_file = ROOT / "profiles" / f"traffic_{SCAN_KERNEL}_gmm_10m_2d.json"
-    if traffic_file.exists():
-        tr = json.loads(traffic_file.read_text())
-        if tr.get("kernel") == SCAN_KERNEL_FULL:
-            traffic = tr.get("hbm_bytes_per_launch")
-    roofline = {
-        "bound": "hbm",
-        "kernel": SCAN_KERNEL_FULL,
-        "achieved": achieved,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS if achieved else None,
-        "traffic": traffic,
-        "algorithmic_bytes": bytes_alg,
-        "kernel_ms": t_kernel * 1e3,
-        "launches_per_step": scan_launches / max(1, args.steps),
-        "valu_frac": (scan_evals / launches * LANE_OPS_PER_EVAL[2] / t_kernel / VALU_LANE_OPS)
-        if t_kernel > 0 else None,
-    }
+    roofline = eb_roofline(st_plain, args.steps)
 
     cpu = None
     if not args.no_cpu:

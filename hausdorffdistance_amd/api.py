@@ -159,6 +159,17 @@ class Context:
     def stream(self) -> int:
         return int(_lib.load().hd_context_stream(self.handle) or 0)
 
+    def set_profiling(self, enable: bool) -> None:
+        """Per-stage device timing on this context's stream."""
+        _lib.check(_lib.load().hd_context_set_profiling(self.handle, 1 if enable else 0))
+
+    def stage_times(self, reset: bool = True) -> dict:
+        """{stage: {"ms", "calls", "evals", "points"}} since the last reset."""
+        buf = C.create_string_buffer(1 << 16)
+        _lib.check(_lib.load().hd_context_stage_times(self.handle, buf, 1 << 16,
+                                                      1 if reset else 0))
+        return json.loads(buf.value.decode()) if buf.value else {}
+
     def close(self) -> None:
         if self._h is not None:
             _lib.load().hd_context_destroy(self._h)

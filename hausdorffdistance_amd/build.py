@@ -58,9 +58,17 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+# Per-file device flags. eb.hip: matrix-core results go to VGPRs (the
+# filter consumes every result with vector integer minima; the AGPR form
+# would add one copy per result).
+FILE_DEVICE = {
+    "eb.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
+
 def _flags_for(src: Path) -> list[str]:
     if src.suffix == ".hip":
-        return COMMON + DEVICE + ["-x", "hip"]
+        return COMMON + DEVICE + FILE_DEVICE.get(src.name, []) + ["-x", "hip"]
     return COMMON + ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-x", "c++"]
 
 

@@ -398,6 +398,9 @@ int hd_directed_device(hd_context* ctx, int variant, int n_dims,
   args.flags = flags;
   args.seed = seed;
   args.model = model;
+  args.has_mbr = true;
+  std::memcpy(args.mbr_a, mbr_a, sizeof(mbr_a));
+  std::memcpy(args.mbr_b, mbr_b, sizeof(mbr_b));
   return TimedDirected(sc.ctx, variant, args, out);
 }
 
@@ -465,9 +468,13 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
   ab.sample_rate = cfg->sample_rate;
   ab.eb_only_threshold = cfg->eb_only_threshold;
   ab.auto_tune = cfg->auto_tune;
+  ab.has_mbr = true;
+  std::memcpy(ab.mbr_a, mbr_a, sizeof(mbr_a));
+  std::memcpy(ab.mbr_b, mbr_b, sizeof(mbr_b));
   DirectedArgs ba = ab;
   std::swap(ba.a, ba.b);
   std::swap(ba.n_a, ba.n_b);
+  std::swap(ba.mbr_a, ba.mbr_b);
   const bool symmetric = (cfg->flags & HD_FLAG_SYMMETRIC) != 0;
 
   std::vector<double> times;

@@ -436,6 +436,374 @@ __global__ void SeedListKernel(const uint32_t* order, Feistel perm,
     out[i] = OrderAt(order_mode, order, perm, i);
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core filter for the budgeted rounds.
+//
+// With coordinates centred on c (p' = fl(p - c)), the 16x16x4 FP32 MFMA
+// computes for 16 B points i and 16 A points j at once
+//   2D: D[i][j] = x'_i (-2 x'_j) + y'_i (-2 y'_j) + |b'_i|^2 * 1 + 1 * |a'_j|^2
+//   3D: D[i][j] = x'_i (-2 x'_j) + y'_i (-2 y'_j) + z'_i (-2 z'_j)
+//                 + (|b'_i|^2 + C) * 1,   C = rho^2
+// i.e. |a - b|^2 (3D: plus C - |a'_j|^2 >= 0, taken back through the
+// threshold), with an error below E = 48 u rho^2 in 2D and 64 u rho^2 in 3D
+// (u = 2^-24, rho = max |p - c| over A and B): products and sums inside the
+// MFMA (at most 2u each, in any order: 8u times the sum of the magnitudes
+// of the four terms, at most 4 rho^2), the norms and C (2-3u each), the
+// centring (8u rho^2) and, in 3D, the threshold arithmetic (10u rho^2).
+// Every D is >= -E, so running minima compare as signed integers on their
+// bit patterns (any negative value is below the positive thresholds, which
+// is all the test needs). A point is dropped only when its
+// approximate running minimum is below T = cmax2 (1 - 2^-19) - E rounded
+// down, which proves its exact FP32 minimum (the reference's unfused
+// formula, within 4u d2 of |a - b|^2) is below cmax2. A step whose
+// approximate minimum for a point lies in [T, cmax2 (1 + 2^-19) + E) may
+// hold an exact hit: that point is then tested exactly against the step's
+// 64 original B points, so points break at the same place as in the exact
+// scan and the filter adds no evaluations. Escalated points carry approx +
+// E rounded up, an upper bound of their exact partial minimum, so a later
+// "below cmax2" test on it is still a proof.
+//
+// A wave holds 64 A points (4 tiles of 16) as MFMA B operands: lane l keeps
+// component l/16 of point l%16 of each tile. A B tile (16 points) is one
+// coalesced 256-byte load, lane l holding component l/16 of point l%16. The
+// output lane l holds D for A point l%16 against B points 4(l/16)..+3, so a
+// point's running minimum lives in 4 lanes of one register per tile. Slot
+// metadata (A index, order position, first step) lives in lane s for slot
+// s = 16 t + j.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+struct MfmaArgs {
+  const float* a;         // AoS A (original coordinates)
+  const uint32_t* order;
+  const float* list_min;  // exact-bound partial minima (skip below cmax2)
+  Feistel perm_a;
+  int order_mode;
+  uint32_t pos_end;
+  unsigned int* cursor;
+  uint32_t claim;         // positions per cursor claim (<= 64)
+  const float* bt;        // B tiles, 64 floats each, 4 per step
+  const float* bx;        // the same points, original coordinates (SoA)
+  const float* by;
+  const float* bz;
+  uint32_t n_steps;
+  uint32_t budget_steps;  // < n_steps: no slot ever completes a full scan
+  float cx, cy, cz;       // centre
+  float err;              // E
+  float off;              // 3D: C
+  unsigned long long* key;
+  uint32_t* esc_idx;
+  float* esc_min;
+  unsigned int* esc_count;
+  unsigned long long* evals;
+};
+
+// Lane's register of one B tile (lane l: component l/16 of point l%16).
+HD_DEV float LoadTile(const float* bt, uint64_t tile, uint32_t lane4) {
+  return LoadRow(bt, tile << 6, lane4);
+}
+
+// One v_min3_i32 (the compiler would split min(a, min(b, c)) into two
+// instructions to shorten the chain on a; here the chain is on one
+// register per tile and every VALU slot counts against the matrix core).
+HD_DEV int32_t Min3I(int32_t a, int32_t b, int32_t c) {
+  int32_t r;
+  asm("v_min3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// Thresholds for the approximate minima from cmax2 bits: below the lower
+// one a point is proved below cmax2, at or above the upper one a step
+// cannot hold an exact hit.
+HD_DEV float FilterThreshold(uint32_t cm_bits, float err) {
+  const double t = (double)__uint_as_float(cm_bits) * (1.0 - 0x1p-19) - (double)err;
+  return t > 0.0 ? __double2float_rd(t) : -kFltMax;
+}
+HD_DEV float FilterUpper(uint32_t cm_bits, float err) {
+  const double t = (double)__uint_as_float(cm_bits) * (1.0 + 0x1p-19) + (double)err;
+  return __double2float_ru(t);
+}
+
+template <int ND>
+__global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
+  const uint32_t lane = LaneId();
+  const uint32_t lane4 = lane << 2;
+  const uint32_t j = lane & 15u;   // A point within a tile
+  const uint32_t kc = lane >> 4;   // operand component
+  const uint32_t wave_id = Uniform((blockIdx.x * kBlock + threadIdx.x) >> 6);
+  const uint32_t n_steps = args.n_steps;
+  uint32_t step = (uint32_t)(((uint64_t)wave_id * 7919u) % n_steps);
+
+  // bop: operand lanes; mn: running minima (FP32 bits as signed integers);
+  // cna: 3D, C - |a'|^2 of the lane's point.
+  float bop[4], cna[4];
+  int32_t mn[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    bop[t] = 0.f;
+    mn[t] = (int32_t)kFltMaxBits;
+    cna[t] = 0.f;
+  }
+  uint32_t m_idx = 0, m_start = 0;  // lane s: slot s
+  float m_ax = 0.f, m_ay = 0.f, m_az = 0.f;  // lane s: slot s, original
+  unsigned long long live = 0;
+  float r_x = 0.f, r_y = 0.f, r_z = 0.f;
+  uint32_t r_idx = 0;
+  uint32_t rhead = 0, rcount = 0;
+  bool drained = false;
+  uint32_t gstep = 0;
+  unsigned long long evals = 0;
+  const uint32_t low_j = (1u << j) - 1u;
+  uint32_t stg_idx = 0, n_stg = 0;
+  float stg_min = 0.f;
+  auto flush = [&]() __attribute__((always_inline)) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(args.esc_count, n_stg);
+    base = Uniform(base);
+    if (lane < n_stg) {
+      args.esc_idx[base + lane] = stg_idx;
+      args.esc_min[base + lane] = stg_min;
+    }
+    n_stg = 0;
+  };
+
+  // Nothing raises cmax2 while a filtered round runs (its points never
+  // finish a scan), so the threshold is fixed for the launch.
+  const uint32_t cm = __float_as_uint(LoadCmax2(args.key));
+  const float cmax2 = __uint_as_float(cm);
+  const float thr = FilterThreshold(cm, args.err);
+  const float thr_hi = FilterUpper(cm, args.err);
+  // An iteration covers two B steps (8 tiles, 32 MFMAs); the next
+  // iteration's tiles and original coordinates are requested at the top of
+  // each iteration.
+  auto next_step = [&](uint32_t st) __attribute__((always_inline)) {
+    return st + 1u == n_steps ? 0u : st + 1u;
+  };
+  float bcur[8], ocur[2][ND];
+  {
+    const uint32_t st1 = next_step(step);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      bcur[q] = LoadTile(args.bt, (uint64_t)Uniform(step) * 4u + q, lane4);
+      bcur[4 + q] = LoadTile(args.bt, (uint64_t)Uniform(st1) * 4u + q, lane4);
+    }
+    LoadStep<ND>(args.bx, args.by, args.bz, step, lane, ocur[0]);
+    LoadStep<ND>(args.bx, args.by, args.bz, st1, lane, ocur[1]);
+  }
+
+  while (true) {
+    // Refill free slots from the compacted reservoir.
+    unsigned long long free_m = ~live;
+    while (free_m != 0ull) {
+      if (rhead == rcount) {
+        if (drained) break;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(args.cursor, args.claim);
+        base = Uniform(base);
+        if (base >= args.pos_end) {
+          drained = true;
+          break;
+        }
+        const uint32_t pos = base + lane;
+        bool valid = lane < args.claim && pos < args.pos_end;
+        if (valid && args.list_min != nullptr)
+          valid = !(__float_as_uint(args.list_min[pos]) < cm);
+        uint32_t idx = 0;
+        float px = 0.f, py = 0.f, pz = 0.f;
+        if (valid) {
+          idx = OrderAt(args.order_mode, args.order, args.perm_a, pos);
+          const float* src = args.a + (uint64_t)idx * ND;
+          px = src[0];
+          py = src[1];
+          if constexpr (ND == 3) pz = src[2];
+        }
+        const unsigned long long vm = __ballot(valid);
+        const uint32_t cnt = (uint32_t)__popcll(vm);
+        const uint32_t below_v = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(vm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)vm, 0u));
+        const uint32_t dest = valid ? below_v : cnt + (lane - below_v);
+        const int d4 = (int)(dest << 2);
+        r_x = __int_as_float(__builtin_amdgcn_ds_permute(d4, __float_as_int(px)));
+        r_y = __int_as_float(__builtin_amdgcn_ds_permute(d4, __float_as_int(py)));
+        if constexpr (ND == 3)
+          r_z = __int_as_float(__builtin_amdgcn_ds_permute(d4, __float_as_int(pz)));
+        r_idx = (uint32_t)__builtin_amdgcn_ds_permute(d4, (int)idx);
+        rhead = 0;
+        rcount = cnt;
+        continue;
+      }
+      const uint32_t navail = rcount - rhead;
+      // Slot s (this lane as metadata holder) takes reservoir entry
+      // rhead + (number of free slots below s) when that is available.
+      const uint32_t rank_s = __builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(free_m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)free_m, 0u));
+      const bool take_s = ((free_m >> lane) & 1ull) && rank_s < navail;
+      const uint32_t src_s = (rhead + rank_s) & 63u;
+      const int src_s4 = (int)(src_s << 2);
+      const uint32_t new_idx = (uint32_t)__builtin_amdgcn_ds_bpermute(src_s4, (int)r_idx);
+      const float new_x = __int_as_float(__builtin_amdgcn_ds_bpermute(src_s4, __float_as_int(r_x)));
+      const float new_y = __int_as_float(__builtin_amdgcn_ds_bpermute(src_s4, __float_as_int(r_y)));
+      float new_z = 0.f;
+      if constexpr (ND == 3)
+        new_z = __int_as_float(__builtin_amdgcn_ds_bpermute(src_s4, __float_as_int(r_z)));
+      if (take_s) {
+        m_idx = new_idx;
+        m_start = gstep;
+        m_ax = new_x;
+        m_ay = new_y;
+        m_az = new_z;
+      }
+      // Operand lanes of each tile.
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint32_t ft = (uint32_t)(free_m >> (16 * t)) & 0xFFFFu;
+        if (ft == 0u) continue;
+        const uint32_t below_t =
+            t == 0 ? 0u : (uint32_t)__popcll(free_m & ((1ull << (16 * t)) - 1ull));
+        const uint32_t rank = below_t + (uint32_t)__builtin_popcount(ft & low_j);
+        const bool take = ((ft >> j) & 1u) && rank < navail;
+        const int src4 = (int)(((rhead + rank) & 63u) << 2);
+        const float x = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(r_x))) - args.cx;
+        const float y = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(r_y))) - args.cy;
+        float z = 0.f, na;
+        if constexpr (ND == 3) {
+          z = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(r_z))) - args.cz;
+          na = x * x + y * y + z * z;
+        } else {
+          na = x * x + y * y;
+        }
+        float val;
+        if constexpr (ND == 2)
+          val = kc == 0u ? -2.f * x : kc == 1u ? -2.f * y : kc == 2u ? 1.f : na;
+        else
+          val = kc == 0u ? -2.f * x : kc == 1u ? -2.f * y : kc == 2u ? -2.f * z : 1.f;
+        if (take) {
+          bop[t] = val;
+          mn[t] = (int32_t)kFltMaxBits;
+          cna[t] = args.off - na;
+        }
+      }
+      const uint32_t took = (uint32_t)__popcll(free_m) < navail ? (uint32_t)__popcll(free_m) : navail;
+      const unsigned long long took_m = __ballot(take_s);
+      live |= took_m;
+      free_m &= ~took_m;
+      rhead += took;
+    }
+    if (live == 0ull) break;
+
+    // Two steps: 8 B tiles against the 4 A tiles (32 MFMAs).
+    const uint32_t s2 = next_step(next_step(step));
+    const uint32_t s3 = next_step(s2);
+    float bnx[8], onx[2][ND];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      bnx[q] = LoadTile(args.bt, (uint64_t)Uniform(s2) * 4u + q, lane4);
+      bnx[4 + q] = LoadTile(args.bt, (uint64_t)Uniform(s3) * 4u + q, lane4);
+    }
+    LoadStep<ND>(args.bx, args.by, args.bz, s2, lane, onx[0]);
+    LoadStep<ND>(args.bx, args.by, args.bz, s3, lane, onx[1]);
+    int32_t sm[4];  // this iteration's approximate minima
+    const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      sm[t] = (int32_t)kFltMaxBits;
+      // A tile without live slots (the cursor has drained) costs nothing.
+      if (((live >> (16 * t)) & 0xFFFFull) == 0ull) continue;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const floatx4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(bcur[q], bop[t], zero, 0, 0, 0);
+        sm[t] = Min3I(sm[t], __float_as_int(d[0]), __float_as_int(d[1]));
+        sm[t] = Min3I(sm[t], __float_as_int(d[2]), __float_as_int(d[3]));
+      }
+      mn[t] = min(mn[t], sm[t]);
+    }
+    gstep += 2;
+    evals += (unsigned long long)__popcll(live) * 128ull;
+
+    // Drops (proved below cmax2, or verified exactly) and escalations.
+    unsigned long long hit = 0, maybe = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float v = __int_as_float(sm[t]);
+      const float lim = ND == 3 ? thr + cna[t] : thr;
+      const float lim_hi = ND == 3 ? thr_hi + cna[t] : thr_hi;
+      unsigned long long c = __ballot(v < lim);
+      unsigned long long m = __ballot(v < lim_hi);
+      c |= c >> 32;
+      c |= c >> 16;
+      m |= m >> 32;
+      m |= m >> 16;
+      hit |= (c & 0xFFFFull) << (16 * t);
+      maybe |= (m & 0xFFFFull) << (16 * t);
+    }
+    hit &= live;
+    maybe &= live & ~hit;
+    while (maybe != 0ull) {
+      // Exact test of this iteration's 128 original B points (the
+      // reference's formula, strict '<').
+      const uint32_t sl = (uint32_t)__builtin_ctzll(maybe);
+      maybe &= maybe - 1ull;
+      const float ax = ReadLaneF(m_ax, sl), ay = ReadLaneF(m_ay, sl);
+      float d0, d1;
+      if constexpr (ND == 2) {
+        d0 = D2(ax, ay, ocur[0][0], ocur[0][1]);
+        d1 = D2(ax, ay, ocur[1][0], ocur[1][1]);
+      } else {
+        const float az = ReadLaneF(m_az, sl);
+        d0 = D2(ax, ay, az, ocur[0][0], ocur[0][1], ocur[0][2]);
+        d1 = D2(ax, ay, az, ocur[1][0], ocur[1][1], ocur[1][2]);
+      }
+      if (WaveAny(d0 < cmax2 || d1 < cmax2)) hit |= 1ull << sl;
+    }
+    unsigned long long due =
+        __ballot(gstep - m_start >= args.budget_steps) & live & ~hit;
+    if (due != 0ull) {
+      // Partial minimum of slot s = 16 t + j: lanes j, j+16, j+32, j+48 of
+      // mn[t]; lane s picks its own tile's value.
+      double mine = 0.0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        int32_t r = min(mn[t], __shfl_xor(mn[t], 16, 64));
+        r = min(r, __shfl_xor(r, 32, 64));
+        double v = (double)__int_as_float(r);
+        if (ND == 3) v -= (double)cna[t];
+        if (kc == (uint32_t)t) mine = v;
+      }
+      // Stage the escalations in lanes [n_stg, n_stg + n_esc) (a forward
+      // permutation: due slots in order, the other lanes after them) and
+      // append 64 at a time, so the shared counter sees one atomic per 64.
+      const uint32_t n_esc = (uint32_t)__popcll(due);
+      if (n_stg + n_esc > (uint32_t)kWaveSize) flush();
+      const bool esc = (due >> lane) & 1ull;
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(due >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)due, 0u));
+      const uint32_t dest =
+          (esc ? n_stg + below : n_stg + n_esc + (lane - below)) & 63u;
+      const double ub = (mine > 0.0 ? mine : 0.0) + (double)args.err;
+      const int d4 = (int)(dest << 2);
+      const uint32_t p_idx = (uint32_t)__builtin_amdgcn_ds_permute(d4, (int)m_idx);
+      const float p_min = __int_as_float(__builtin_amdgcn_ds_permute(
+          d4, __float_as_int(__double2float_ru(ub))));
+      if (lane >= n_stg && lane < n_stg + n_esc) {
+        stg_idx = p_idx;
+        stg_min = p_min;
+      }
+      n_stg += n_esc;
+      if (n_stg == (uint32_t)kWaveSize) flush();
+    }
+    live &= ~(hit | due);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) bcur[q] = bnx[q];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      ocur[0][d] = onx[0][d];
+      ocur[1][d] = onx[1][d];
+    }
+    step = s2;
+  }
+  if (n_stg > 0) flush();
+  if (lane == 0) atomicAdd(args.evals, evals);
+}
+
 constexpr int kP = 8;  // slots per wave
 constexpr int kS = 4;  // B steps per iteration
 
@@ -555,15 +923,154 @@ int LaunchSlots(hd_context* ctx, const float* a, int order_mode,
   return LaunchScan<ND>(ctx, args, pos_begin, pos_end - pos_begin);
 }
 
-int ReadCount(hd_context* ctx, const unsigned int* d_count, uint32_t* out) {
+// Centred B tiles for the matrix-core filter: tile q holds points
+// 16q..16q+15 of the shuffled SoA copy as [x' x16, y' x16, |b'|^2 x16, 1 x16]
+// (2D) or [x', y', z', |b'|^2 + C] (3D).
+template <int ND>
+__global__ void BuildTilesKernel(const float* __restrict__ bx,
+                                 const float* __restrict__ by,
+                                 const float* __restrict__ bz, uint32_t n_bp,
+                                 float cx, float cy, float cz, float off,
+                                 float* bt) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_bp;
+       p += gridDim.x * blockDim.x) {
+    const float x = bx[p] - cx, y = by[p] - cy;
+    const uint64_t o = (uint64_t)(p >> 4) * 64u + (p & 15u);
+    bt[o] = x;
+    bt[o + 16] = y;
+    if constexpr (ND == 2) {
+      bt[o + 32] = x * x + y * y;
+      bt[o + 48] = 1.f;
+    } else {
+      const float z = bz[p] - cz;
+      bt[o + 32] = z;
+      bt[o + 48] = (x * x + y * y + z * z) + off;
+    }
+  }
+}
+
+// Centre, error bound and tiles of the filter (see EbMfmaKernel). Leaves
+// t->bt null when the coordinates are too large for FP32 norms.
+template <int ND>
+int PrepareFilter(hd_context* ctx, const float* mbr_a, const float* mbr_b,
+                  EbTarget* t) {
+  double rho2 = 0.0;
+  for (int d = 0; d < ND; ++d) {
+    const double lo = std::min(mbr_a[d], mbr_b[d]);
+    const double hi = std::max(mbr_a[ND + d], mbr_b[ND + d]);
+    const float c = (float)((lo + hi) * 0.5);
+    const double r = std::max(std::fabs(lo - (double)c), std::fabs(hi - (double)c));
+    t->center[d] = c;
+    rho2 += r * r;
+  }
+  rho2 *= 1.0 + 0x1p-20;
+  if (!(rho2 < 1e30)) {
+    t->bt = nullptr;
+    return HD_OK;
+  }
+  t->err = (float)((ND == 2 ? 48.0 : 64.0) * 0x1p-24 * rho2 * (1.0 + 0x1p-20) +
+                   0x1p-100);
+  t->off = ND == 3 ? (float)(rho2 * (1.0 + 0x1p-20)) : 0.f;
+  HD_TRY(ContextArray(ctx, "eb_tiles", (uint64_t)t->n_bp * 4u, &t->bt));
+  const int grid = (int)std::min<uint64_t>((t->n_bp + 255u) / 256u,
+                                           (uint64_t)ctx->cu_count * 8u);
+  BuildTilesKernel<ND><<<grid, 256, 0, ctx->stream>>>(
+      t->bx, t->by, t->bz, t->n_bp, t->center[0], t->center[1], t->center[2],
+      t->off, t->bt);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
+
+template <int ND>
+int MfmaResidentBlocks(hd_context* ctx) {
+  static int per_cu[4] = {0, 0, 0, 0};
+  if (per_cu[ND] == 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, EbMfmaKernel<ND>, kBlock, 0) != hipSuccess || n < 1)
+      n = 1;
+    per_cu[ND] = n;
+  }
+  return per_cu[ND] * ctx->cu_count;
+}
+
+// Filtered budgeted round over order positions [pos_begin, pos_end); every
+// point that is not proved below cmax2 within budget_steps is escalated
+// with an upper bound of its partial minimum.
+template <int ND>
+int LaunchMfma(hd_context* ctx, const float* a, int order_mode,
+               const uint32_t* order, const float* list_min,
+               const Feistel& perm_a, uint32_t pos_begin, uint32_t pos_end,
+               const EbTarget& t, uint32_t budget_steps,
+               unsigned long long* key, unsigned long long* evals,
+               unsigned int* counters, uint32_t* esc_idx, float* esc_min) {
+  if (pos_begin >= pos_end) return HD_OK;
+  HD_TRY(FillU32(ctx, counters, pos_begin, 1));
+  HD_TRY(FillU32(ctx, counters + 1, 0u, 1));
+  MfmaArgs args = {};
+  args.a = a;
+  args.order = order;
+  args.list_min = list_min;
+  args.perm_a = perm_a;
+  args.order_mode = order_mode;
+  args.pos_end = pos_end;
+  args.cursor = counters;
+  args.bt = t.bt;
+  args.bx = t.bx;
+  args.by = t.by;
+  args.bz = t.bz;
+  args.n_steps = t.n_steps;
+  args.budget_steps = budget_steps;
+  args.cx = t.center[0];
+  args.cy = t.center[1];
+  args.cz = t.center[2];
+  args.err = t.err;
+  args.off = t.off;
+  args.key = key;
+  args.esc_idx = esc_idx;
+  args.esc_min = esc_min;
+  args.esc_count = counters + 1;
+  args.evals = evals;
+  const uint32_t n_entries = pos_end - pos_begin;
+  int blocks = MfmaResidentBlocks<ND>(ctx);
+  const uint32_t waves_needed = (n_entries + 63u) / 64u;
+  if ((uint32_t)blocks * 4u > waves_needed)
+    blocks = (int)std::max(1u, (waves_needed + 3u) / 4u);
+  const uint32_t n_waves = (uint32_t)blocks * (kBlock / kWaveSize);
+  args.claim = std::min<uint32_t>(
+      kWaveSize, std::max<uint32_t>(16u, (n_entries + n_waves - 1) / n_waves));
+  EbMfmaKernel<ND><<<blocks, kBlock, 0, ctx->stream>>>(args);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
+
+// Reads a device counter and, optionally, the current cmax2 (one sync).
+int ReadCount(hd_context* ctx, const unsigned int* d_count, uint32_t* out,
+              const unsigned long long* key = nullptr, float* cmax2 = nullptr) {
   void* pinned = nullptr;
   HD_TRY(ContextPinned(ctx, 64, &pinned));
   unsigned int* h = static_cast<unsigned int*>(pinned);
-  HD_HIP_TRY(hipMemcpyAsync(h, d_count, 4, hipMemcpyDeviceToHost,
-                            ctx->stream));
+  if (d_count != nullptr)
+    HD_HIP_TRY(hipMemcpyAsync(h, d_count, 4, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  if (key != nullptr)
+    HD_HIP_TRY(hipMemcpyAsync(h + 2, key, 8, hipMemcpyDeviceToHost,
+                              ctx->stream));
   HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  *out = *h;
+  if (out != nullptr) *out = h[0];
+  if (cmax2 != nullptr) {
+    float f;
+    std::memcpy(&f, &h[3], 4);
+    *cmax2 = f;
+  }
   return HD_OK;
+}
+
+// The filter pays off while steps that need an exact test stay rare, i.e.
+// while its error bound is small next to cmax2.
+bool UseFilter(const EbTarget& t, uint32_t budget, float cmax2) {
+  return t.bt != nullptr && budget < t.n_steps && cmax2 > 0.f &&
+         t.err < 0.5f * cmax2;
 }
 
 // B repacked for the scans: SoA, pseudo-randomly permuted (unless the
@@ -609,13 +1116,26 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
   int mode = order_mode;
   const uint32_t* list = order;
   const float* list_min = order_min;
+  float cmax2 = 0.f;
   for (int r = 0; r < n_rounds && begin < end; ++r) {
     const uint32_t budget = first_budget ? first_budget : kBudgets[r];
     rs->round_points[r] += end - begin;
-    const int st_slot = StageBegin(ctx, r == 0 ? "eb_slot" : "eb_slot_more");
-    HD_TRY(LaunchSlots<ND>(ctx, a, mode, list, list_min, perm_a, begin, end,
-                           t.bx, t.by, t.bz, t.n_steps, budget, key, ev + r,
-                           counters, idx_out, min_out));
+    if (t.bt != nullptr && budget < t.n_steps)
+      HD_TRY(ReadCount(ctx, nullptr, nullptr, key, &cmax2));
+    // Matrix-core filter when its threshold keeps most of cmax2, else the
+    // exact scan.
+    const bool filt = UseFilter(t, budget, cmax2);
+    if (filt) rs->mfma_rounds |= 1u << r;
+    const int st_slot = StageBegin(
+        ctx, filt ? (r == 0 ? "eb_mfma" : "eb_mfma_more")
+                  : (r == 0 ? "eb_slot" : "eb_slot_more"));
+    if (filt)
+      HD_TRY(LaunchMfma<ND>(ctx, a, mode, list, list_min, perm_a, begin, end, t,
+                            budget, key, ev + r, counters, idx_out, min_out));
+    else
+      HD_TRY(LaunchSlots<ND>(ctx, a, mode, list, list_min, perm_a, begin, end,
+                             t.bx, t.by, t.bz, t.n_steps, budget, key, ev + r,
+                             counters, idx_out, min_out));
     StageEnd(ctx, st_slot);
     uint32_t n_esc = 0;
     HD_TRY(ReadCount(ctx, counters + 1, &n_esc));
@@ -670,6 +1190,8 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
   const int st_prep = StageBegin(ctx, "eb_prep");
   EbTarget t;
   HD_TRY(PrepareB<ND>(ctx, in.b, n_b, shuffle, in.seed, &t));
+  if (in.has_mbr && t.n_steps > 16u)
+    HD_TRY(PrepareFilter<ND>(ctx, in.mbr_a, in.mbr_b, &t));
   // Visit order of A: learned (array), pseudo-random (Feistel) or input.
   const uint32_t* order = nullptr;
   const Feistel perm_a = MakeFeistel(n_a, (uint64_t)(int64_t)in.seed, 0);
@@ -687,8 +1209,13 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
   StageEnd(ctx, st_prep);
 
   // Stage 1: exact minima of the first s0 points of the order.
+  // With a learned order the first points are the ones predicted farthest
+  // from B; completing 2048 of them usually reaches the final cmax2 before
+  // the rounds start. A pseudo-random order gains little from more than a
+  // few hundred.
+  const uint32_t s0_default = order_mode == kOrderArray ? 2048u : 256u;
   const uint32_t s0 = std::min<uint32_t>(
-      n_a, in.seed_points ? in.seed_points : 256u);
+      n_a, in.seed_points ? in.seed_points : s0_default);
   uint32_t* seed_list = nullptr;
   HD_TRY(ContextArray(ctx, "eb_seed_list", s0, &seed_list));
   const int st_seed = StageBegin(ctx, "eb_seed");
@@ -713,9 +1240,12 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
     HD_HIP_TRY(hipMemcpyAsync(h, ev, 48, hipMemcpyDeviceToHost, ctx->stream));
     HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
     StageCount(ctx, "eb_seed", h[0], s0);
-    StageCount(ctx, "eb_slot", h[1], rs.round_points[0]);
-    StageCount(ctx, "eb_slot_more", h[2] + h[3],
-               rs.round_points[1] + rs.round_points[2]);
+    for (int r = 0; r < 3; ++r) {
+      const bool filt = (rs.mfma_rounds >> r) & 1u;
+      StageCount(ctx, filt ? (r == 0 ? "eb_mfma" : "eb_mfma_more")
+                           : (r == 0 ? "eb_slot" : "eb_slot_more"),
+                 h[1 + r], rs.round_points[r]);
+    }
     StageCount(ctx, "eb_top", h[4], rs.top_points);
     StageCount(ctx, "eb_tail", h[5], rs.tail_points);
   }

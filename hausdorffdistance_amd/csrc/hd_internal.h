@@ -160,6 +160,11 @@ struct DirectedArgs {
   float sample_rate = 0.f;
   uint32_t eb_only_threshold = 0;
   int auto_tune = 0;
+  // Bounding boxes (lo[nd], hi[nd]) of A and B when the caller measured
+  // them; they enable the matrix-core filter of the early-break rounds.
+  bool has_mbr = false;
+  float mbr_a[6] = {0, 0, 0, 0, 0, 0};
+  float mbr_b[6] = {0, 0, 0, 0, 0, 0};
 };
 
 int RunEarlyBreak(hd_context* ctx, const DirectedArgs& args,
@@ -183,9 +188,16 @@ struct EbTarget {
   float* bz = nullptr;
   uint32_t n_bp = 0;     // padded to whole 64-point steps
   uint32_t n_steps = 0;  // n_bp / 64
+  // Matrix-core filter (null when not prepared): centred B tiles of 16
+  // points, the centre and the error bound of the approximate d2.
+  float* bt = nullptr;
+  float center[3] = {0, 0, 0};
+  float err = 0.f;
+  float off = 0.f;  // 3D: C, which keeps the approximate values >= -err
 };
 struct EbRoundStats {
   uint64_t round_points[3] = {0, 0, 0};
+  uint32_t mfma_rounds = 0;
   uint64_t top_points = 0;
   uint64_t tail_points = 0;
 };

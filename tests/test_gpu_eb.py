@@ -108,3 +108,58 @@ def test_check_flag_runs_plain_eb(ctx):
     r = run(a, b, ctx=ctx, check=True)
     assert r.check_pass == 1
     assert r.stats["Check"]["Pass"] is True
+
+
+def _eb_with_stages(ctx, a, b, **kw):
+    ctx.set_profiling(True)
+    ctx.stage_times(reset=True)
+    try:
+        r = run(a, b, ctx=ctx, **kw)
+        return r, ctx.stage_times(reset=True)
+    finally:
+        ctx.set_profiling(False)
+
+
+@pytest.mark.parametrize("case", ["gmm_2d", "sphere_3d"])
+def test_matrix_core_filter_rounds_are_exact(ctx, case):
+    # Rounds run on the matrix-core filter (approximate d2 with a proved
+    # error bound); the published bits still come from exact scans.
+    if case == "gmm_2d":
+        a, b = hd.gen.gmm_pair(200_000)
+    else:
+        a = hd.gen.sphere(17, 120_000, 0.005, 0.0)
+        b = hd.gen.sphere(18, 90_000, 0.005, 1.0)
+    print(f"[filter] {case}", flush=True)
+    r, st = _eb_with_stages(ctx, a, b)
+    assert "eb_mfma" in st and st["eb_mfma"]["calls"] >= 1, sorted(st)
+    o_ab, o_ba = oracle.eb_symmetric(a, b)
+    assert triple(r.ab) == triple(o_ab)
+    assert triple(r.ba) == triple(o_ba)
+
+
+def test_matrix_core_filter_with_all_points_tied(ctx):
+    # Every A point's nearest neighbour is at exactly the same distance: no
+    # point may be dropped by the approximate test, all of them reach the
+    # exact scans, and the smallest index wins.
+    g = np.arange(96, dtype=np.float32) * np.float32(2.0 ** -7)
+    xx, yy = np.meshgrid(g, g, indexing="ij")
+    a = np.stack([xx.ravel(), yy.ravel()], 1).astype(np.float32)
+    b = a + np.array([2.0 ** -9, 0.0], np.float32)
+    print("[filter] tied lattice", a.shape, flush=True)
+    r, st = _eb_with_stages(ctx, a, b, symmetric=False)
+    o = oracle.brute_directed(a, b)
+    assert triple(r.ab) == triple(o)
+    assert o.argmax == 0 and r.ab.cmax2_bits == np.float32(2.0 ** -18).view(np.uint32)
+
+
+def test_matrix_core_filter_far_from_origin(ctx):
+    # A large common offset: the filter centres coordinates before the
+    # matrix products; results must not change.
+    a, b = hd.gen.gmm_pair(100_000)
+    off = np.array([1000.0, -250.0], np.float32)
+    a2, b2 = a + off, b + off
+    print("[filter] offset", flush=True)
+    r, st = _eb_with_stages(ctx, a2, b2)
+    o_ab, o_ba = oracle.eb_symmetric(a2, b2)
+    assert triple(r.ab) == triple(o_ab)
+    assert triple(r.ba) == triple(o_ba)
