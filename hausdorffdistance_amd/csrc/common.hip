@@ -245,9 +245,9 @@ int FinishDirected(hd_context* ctx, int n_dims, const float* a,
                    const unsigned long long* key_dev,
                    const unsigned long long* evals_dev, int n_counters,
                    hd_directed_result* out) {
-  if (n_counters > 8) return Fail(HD_EINVAL, "too many counters");
+  if (n_counters > 16) return Fail(HD_EINVAL, "too many counters");
   void* pinned = nullptr;
-  HD_TRY(ContextPinned(ctx, 128, &pinned));
+  HD_TRY(ContextPinned(ctx, 256, &pinned));
   unsigned long long* h = static_cast<unsigned long long*>(pinned);
   HD_HIP_TRY(hipMemcpyAsync(&h[0], key_dev, 8, hipMemcpyDeviceToHost,
                             ctx->stream));

@@ -479,8 +479,14 @@ struct MfmaArgs {
   Feistel perm_a;
   int order_mode;
   uint32_t pos_end;
-  unsigned int* cursor;
+  unsigned int* cursors;  // per segment: next order position
   uint32_t claim;         // positions per cursor claim (<= 64)
+  uint32_t seg_steps;     // steps per segment
+  uint32_t n_segs;        // 1: budgeted round over all of B
+  uint32_t* broken;       // n_segs > 1, per position: 1 once a hit is known
+  uint32_t* gdone;        // n_segs > 1, per position: segments completed
+  uint32_t* cand;         // n_segs > 1: A indices that completed every
+  unsigned int* cand_count;  // segment without a hit (exact scan follows)
   const float* bt;        // B tiles, 64 floats each, 4 per step
   const float* bx;        // the same points, original coordinates (SoA)
   const float* by;
@@ -530,8 +536,9 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
   const uint32_t j = lane & 15u;   // A point within a tile
   const uint32_t kc = lane >> 4;   // operand component
   const uint32_t wave_id = Uniform((blockIdx.x * kBlock + threadIdx.x) >> 6);
+  const uint32_t n_waves = gridDim.x * (kBlock / kWaveSize);
   const uint32_t n_steps = args.n_steps;
-  uint32_t step = (uint32_t)(((uint64_t)wave_id * 7919u) % n_steps);
+  const bool split = args.n_segs > 1;
 
   // bop: operand lanes; mn: running minima (FP32 bits as signed integers);
   // cna: 3D, C - |a'|^2 of the lane's point.
@@ -543,11 +550,11 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
     mn[t] = (int32_t)kFltMaxBits;
     cna[t] = 0.f;
   }
-  uint32_t m_idx = 0, m_start = 0;  // lane s: slot s
+  uint32_t m_idx = 0, m_start = 0, m_pos = 0;  // lane s: slot s
   float m_ax = 0.f, m_ay = 0.f, m_az = 0.f;  // lane s: slot s, original
   unsigned long long live = 0;
   float r_x = 0.f, r_y = 0.f, r_z = 0.f;
-  uint32_t r_idx = 0;
+  uint32_t r_idx = 0, r_pos = 0;
   uint32_t rhead = 0, rcount = 0;
   bool drained = false;
   uint32_t gstep = 0;
@@ -572,233 +579,280 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
   const float cmax2 = __uint_as_float(cm);
   const float thr = FilterThreshold(cm, args.err);
   const float thr_hi = FilterUpper(cm, args.err);
-  // An iteration covers two B steps (8 tiles, 32 MFMAs); the next
-  // iteration's tiles and original coordinates are requested at the top of
-  // each iteration.
-  auto next_step = [&](uint32_t st) __attribute__((always_inline)) {
-    return st + 1u == n_steps ? 0u : st + 1u;
-  };
-  float bcur[8], ocur[2][ND];
-  {
-    const uint32_t st1 = next_step(step);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      bcur[q] = LoadTile(args.bt, (uint64_t)Uniform(step) * 4u + q, lane4);
-      bcur[4 + q] = LoadTile(args.bt, (uint64_t)Uniform(st1) * 4u + q, lane4);
+  // Wave k of the sweep takes segment k % n_segs (one segment, all of B, in
+  // a budgeted round); with fewer waves than segments a wave takes
+  // segments wave_id, wave_id + n_waves, ...
+  const uint32_t sweep = n_waves > args.n_segs ? n_waves : args.n_segs;
+  for (uint32_t kk = wave_id; kk < sweep; kk += n_waves) {
+    const uint32_t seg = kk % args.n_segs;
+    const uint32_t seg_begin = seg * args.seg_steps;
+    const uint32_t seg_len = min(args.seg_steps, n_steps - seg_begin);
+    const uint32_t seg_end = seg_begin + seg_len;
+    // Budgeted rounds escalate after budget_steps (< n_steps); segments are
+    // complete after seg_len steps.
+    const uint32_t limit = split ? seg_len : args.budget_steps;
+    unsigned int* cursor = args.cursors + seg;
+    uint32_t step =
+        seg_begin + (uint32_t)(((uint64_t)(kk / args.n_segs) * 7919u) % seg_len);
+    live = 0;
+    rhead = rcount = 0;
+    drained = false;
+    gstep = 0;
+    // An iteration covers two B steps (8 tiles, 32 MFMAs); the next
+    // iteration's tiles and original coordinates are requested at the top of
+    // each iteration.
+    auto next_step = [&](uint32_t st) __attribute__((always_inline)) {
+      return st + 1u == seg_end ? seg_begin : st + 1u;
+    };
+    float bcur[8], ocur[2][ND];
+    {
+      const uint32_t st1 = next_step(step);
+  #pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        bcur[q] = LoadTile(args.bt, (uint64_t)Uniform(step) * 4u + q, lane4);
+        bcur[4 + q] = LoadTile(args.bt, (uint64_t)Uniform(st1) * 4u + q, lane4);
+      }
+      LoadStep<ND>(args.bx, args.by, args.bz, step, lane, ocur[0]);
+      LoadStep<ND>(args.bx, args.by, args.bz, st1, lane, ocur[1]);
     }
-    LoadStep<ND>(args.bx, args.by, args.bz, step, lane, ocur[0]);
-    LoadStep<ND>(args.bx, args.by, args.bz, st1, lane, ocur[1]);
-  }
 
-  while (true) {
-    // Refill free slots from the compacted reservoir.
-    unsigned long long free_m = ~live;
-    while (free_m != 0ull) {
-      if (rhead == rcount) {
-        if (drained) break;
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(args.cursor, args.claim);
-        base = Uniform(base);
-        if (base >= args.pos_end) {
-          drained = true;
-          break;
+    while (true) {
+      // Refill free slots from the compacted reservoir.
+      unsigned long long free_m = ~live;
+      while (free_m != 0ull) {
+        if (rhead == rcount) {
+          if (drained) break;
+          uint32_t base = 0;
+          if (lane == 0) base = atomicAdd(cursor, args.claim);
+          base = Uniform(base);
+          if (base >= args.pos_end) {
+            drained = true;
+            break;
+          }
+          const uint32_t pos = base + lane;
+          bool valid = lane < args.claim && pos < args.pos_end;
+          if (valid && args.list_min != nullptr)
+            valid = !(__float_as_uint(args.list_min[pos]) < cm);
+          if (valid && split)
+            valid = __hip_atomic_load(&args.broken[pos], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) == 0u;
+          uint32_t idx = 0;
+          float px = 0.f, py = 0.f, pz = 0.f;
+          if (valid) {
+            idx = OrderAt(args.order_mode, args.order, args.perm_a, pos);
+            const float* src = args.a + (uint64_t)idx * ND;
+            px = src[0];
+            py = src[1];
+            if constexpr (ND == 3) pz = src[2];
+          }
+          const unsigned long long vm = __ballot(valid);
+          const uint32_t cnt = (uint32_t)__popcll(vm);
+          const uint32_t below_v = __builtin_amdgcn_mbcnt_hi(
+              (uint32_t)(vm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)vm, 0u));
+          const uint32_t dest = valid ? below_v : cnt + (lane - below_v);
+          const int d4 = (int)(dest << 2);
+          r_x = __int_as_float(__builtin_amdgcn_ds_permute(d4, __float_as_int(px)));
+          r_y = __int_as_float(__builtin_amdgcn_ds_permute(d4, __float_as_int(py)));
+          if constexpr (ND == 3)
+            r_z = __int_as_float(__builtin_amdgcn_ds_permute(d4, __float_as_int(pz)));
+          r_idx = (uint32_t)__builtin_amdgcn_ds_permute(d4, (int)idx);
+          r_pos = (uint32_t)__builtin_amdgcn_ds_permute(d4, (int)pos);
+          rhead = 0;
+          rcount = cnt;
+          continue;
         }
-        const uint32_t pos = base + lane;
-        bool valid = lane < args.claim && pos < args.pos_end;
-        if (valid && args.list_min != nullptr)
-          valid = !(__float_as_uint(args.list_min[pos]) < cm);
-        uint32_t idx = 0;
-        float px = 0.f, py = 0.f, pz = 0.f;
-        if (valid) {
-          idx = OrderAt(args.order_mode, args.order, args.perm_a, pos);
-          const float* src = args.a + (uint64_t)idx * ND;
-          px = src[0];
-          py = src[1];
-          if constexpr (ND == 3) pz = src[2];
-        }
-        const unsigned long long vm = __ballot(valid);
-        const uint32_t cnt = (uint32_t)__popcll(vm);
-        const uint32_t below_v = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(vm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)vm, 0u));
-        const uint32_t dest = valid ? below_v : cnt + (lane - below_v);
-        const int d4 = (int)(dest << 2);
-        r_x = __int_as_float(__builtin_amdgcn_ds_permute(d4, __float_as_int(px)));
-        r_y = __int_as_float(__builtin_amdgcn_ds_permute(d4, __float_as_int(py)));
+        const uint32_t navail = rcount - rhead;
+        // Slot s (this lane as metadata holder) takes reservoir entry
+        // rhead + (number of free slots below s) when that is available.
+        const uint32_t rank_s = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(free_m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)free_m, 0u));
+        const bool take_s = ((free_m >> lane) & 1ull) && rank_s < navail;
+        const uint32_t src_s = (rhead + rank_s) & 63u;
+        const int src_s4 = (int)(src_s << 2);
+        const uint32_t new_idx = (uint32_t)__builtin_amdgcn_ds_bpermute(src_s4, (int)r_idx);
+        const uint32_t new_pos = (uint32_t)__builtin_amdgcn_ds_bpermute(src_s4, (int)r_pos);
+        const float new_x = __int_as_float(__builtin_amdgcn_ds_bpermute(src_s4, __float_as_int(r_x)));
+        const float new_y = __int_as_float(__builtin_amdgcn_ds_bpermute(src_s4, __float_as_int(r_y)));
+        float new_z = 0.f;
         if constexpr (ND == 3)
-          r_z = __int_as_float(__builtin_amdgcn_ds_permute(d4, __float_as_int(pz)));
-        r_idx = (uint32_t)__builtin_amdgcn_ds_permute(d4, (int)idx);
-        rhead = 0;
-        rcount = cnt;
-        continue;
+          new_z = __int_as_float(__builtin_amdgcn_ds_bpermute(src_s4, __float_as_int(r_z)));
+        if (take_s) {
+          m_idx = new_idx;
+          m_pos = new_pos;
+          m_start = gstep;
+          m_ax = new_x;
+          m_ay = new_y;
+          m_az = new_z;
+        }
+        // Operand lanes of each tile.
+  #pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const uint32_t ft = (uint32_t)(free_m >> (16 * t)) & 0xFFFFu;
+          if (ft == 0u) continue;
+          const uint32_t below_t =
+              t == 0 ? 0u : (uint32_t)__popcll(free_m & ((1ull << (16 * t)) - 1ull));
+          const uint32_t rank = below_t + (uint32_t)__builtin_popcount(ft & low_j);
+          const bool take = ((ft >> j) & 1u) && rank < navail;
+          const int src4 = (int)(((rhead + rank) & 63u) << 2);
+          const float x = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(r_x))) - args.cx;
+          const float y = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(r_y))) - args.cy;
+          float z = 0.f, na;
+          if constexpr (ND == 3) {
+            z = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(r_z))) - args.cz;
+            na = x * x + y * y + z * z;
+          } else {
+            na = x * x + y * y;
+          }
+          float val;
+          if constexpr (ND == 2)
+            val = kc == 0u ? -2.f * x : kc == 1u ? -2.f * y : kc == 2u ? 1.f : na;
+          else
+            val = kc == 0u ? -2.f * x : kc == 1u ? -2.f * y : kc == 2u ? -2.f * z : 1.f;
+          if (take) {
+            bop[t] = val;
+            mn[t] = (int32_t)kFltMaxBits;
+            cna[t] = args.off - na;
+          }
+        }
+        const uint32_t took = (uint32_t)__popcll(free_m) < navail ? (uint32_t)__popcll(free_m) : navail;
+        const unsigned long long took_m = __ballot(take_s);
+        live |= took_m;
+        free_m &= ~took_m;
+        rhead += took;
       }
-      const uint32_t navail = rcount - rhead;
-      // Slot s (this lane as metadata holder) takes reservoir entry
-      // rhead + (number of free slots below s) when that is available.
-      const uint32_t rank_s = __builtin_amdgcn_mbcnt_hi(
-          (uint32_t)(free_m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)free_m, 0u));
-      const bool take_s = ((free_m >> lane) & 1ull) && rank_s < navail;
-      const uint32_t src_s = (rhead + rank_s) & 63u;
-      const int src_s4 = (int)(src_s << 2);
-      const uint32_t new_idx = (uint32_t)__builtin_amdgcn_ds_bpermute(src_s4, (int)r_idx);
-      const float new_x = __int_as_float(__builtin_amdgcn_ds_bpermute(src_s4, __float_as_int(r_x)));
-      const float new_y = __int_as_float(__builtin_amdgcn_ds_bpermute(src_s4, __float_as_int(r_y)));
-      float new_z = 0.f;
-      if constexpr (ND == 3)
-        new_z = __int_as_float(__builtin_amdgcn_ds_bpermute(src_s4, __float_as_int(r_z)));
-      if (take_s) {
-        m_idx = new_idx;
-        m_start = gstep;
-        m_ax = new_x;
-        m_ay = new_y;
-        m_az = new_z;
+      if (live == 0ull) break;
+
+      // Two steps: 8 B tiles against the 4 A tiles (32 MFMAs).
+      const uint32_t s2 = next_step(next_step(step));
+      const uint32_t s3 = next_step(s2);
+      float bnx[8], onx[2][ND];
+  #pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        bnx[q] = LoadTile(args.bt, (uint64_t)Uniform(s2) * 4u + q, lane4);
+        bnx[4 + q] = LoadTile(args.bt, (uint64_t)Uniform(s3) * 4u + q, lane4);
       }
-      // Operand lanes of each tile.
-#pragma unroll
+      LoadStep<ND>(args.bx, args.by, args.bz, s2, lane, onx[0]);
+      LoadStep<ND>(args.bx, args.by, args.bz, s3, lane, onx[1]);
+      // Segmented scans poll the other segments' verdicts every 8 steps.
+      const bool poll = split && (gstep & 7u) == 6u;
+      uint32_t flag = 0;
+      if (poll && ((live >> lane) & 1ull))
+        flag = __hip_atomic_load(&args.broken[m_pos], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+      int32_t sm[4];  // this iteration's approximate minima
+      const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+  #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const uint32_t ft = (uint32_t)(free_m >> (16 * t)) & 0xFFFFu;
-        if (ft == 0u) continue;
-        const uint32_t below_t =
-            t == 0 ? 0u : (uint32_t)__popcll(free_m & ((1ull << (16 * t)) - 1ull));
-        const uint32_t rank = below_t + (uint32_t)__builtin_popcount(ft & low_j);
-        const bool take = ((ft >> j) & 1u) && rank < navail;
-        const int src4 = (int)(((rhead + rank) & 63u) << 2);
-        const float x = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(r_x))) - args.cx;
-        const float y = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(r_y))) - args.cy;
-        float z = 0.f, na;
-        if constexpr (ND == 3) {
-          z = __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(r_z))) - args.cz;
-          na = x * x + y * y + z * z;
+        sm[t] = (int32_t)kFltMaxBits;
+        // A tile without live slots (the cursor has drained) costs nothing.
+        if (((live >> (16 * t)) & 0xFFFFull) == 0ull) continue;
+  #pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const floatx4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(bcur[q], bop[t], zero, 0, 0, 0);
+          sm[t] = Min3I(sm[t], __float_as_int(d[0]), __float_as_int(d[1]));
+          sm[t] = Min3I(sm[t], __float_as_int(d[2]), __float_as_int(d[3]));
+        }
+        mn[t] = min(mn[t], sm[t]);
+      }
+      gstep += 2;
+      evals += (unsigned long long)__popcll(live) * 128ull;
+
+      // Drops (proved below cmax2, or verified exactly) and escalations.
+      unsigned long long hit = 0, maybe = 0;
+  #pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float v = __int_as_float(sm[t]);
+        const float lim = ND == 3 ? thr + cna[t] : thr;
+        const float lim_hi = ND == 3 ? thr_hi + cna[t] : thr_hi;
+        unsigned long long c = __ballot(v < lim);
+        unsigned long long m = __ballot(v < lim_hi);
+        c |= c >> 32;
+        c |= c >> 16;
+        m |= m >> 32;
+        m |= m >> 16;
+        hit |= (c & 0xFFFFull) << (16 * t);
+        maybe |= (m & 0xFFFFull) << (16 * t);
+      }
+      hit &= live;
+      maybe &= live & ~hit;
+      while (maybe != 0ull) {
+        // Exact test of this iteration's 128 original B points (the
+        // reference's formula, strict '<').
+        const uint32_t sl = (uint32_t)__builtin_ctzll(maybe);
+        maybe &= maybe - 1ull;
+        const float ax = ReadLaneF(m_ax, sl), ay = ReadLaneF(m_ay, sl);
+        float d0, d1;
+        if constexpr (ND == 2) {
+          d0 = D2(ax, ay, ocur[0][0], ocur[0][1]);
+          d1 = D2(ax, ay, ocur[1][0], ocur[1][1]);
         } else {
-          na = x * x + y * y;
+          const float az = ReadLaneF(m_az, sl);
+          d0 = D2(ax, ay, az, ocur[0][0], ocur[0][1], ocur[0][2]);
+          d1 = D2(ax, ay, az, ocur[1][0], ocur[1][1], ocur[1][2]);
         }
-        float val;
-        if constexpr (ND == 2)
-          val = kc == 0u ? -2.f * x : kc == 1u ? -2.f * y : kc == 2u ? 1.f : na;
-        else
-          val = kc == 0u ? -2.f * x : kc == 1u ? -2.f * y : kc == 2u ? -2.f * z : 1.f;
-        if (take) {
-          bop[t] = val;
-          mn[t] = (int32_t)kFltMaxBits;
-          cna[t] = args.off - na;
+        if (WaveAny(d0 < cmax2 || d1 < cmax2)) hit |= 1ull << sl;
+      }
+      if (split && hit != 0ull) {
+        // Tell the other segments' waves this entry is decided.
+        if ((hit >> lane) & 1ull)
+          __hip_atomic_store(&args.broken[m_pos], 1u, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      }
+      const unsigned long long gone =
+          poll ? __ballot(flag != 0u) & live & ~hit : 0ull;
+      unsigned long long due =
+          __ballot(gstep - m_start >= limit) & live & ~(hit | gone);
+      if (split && due != 0ull) {
+        // Segment scanned without a hit: count it; an entry that completes
+        // every segment goes to the exact scan.
+        if ((due >> lane) & 1ull) {
+          const uint32_t prev = atomicAdd(&args.gdone[m_pos], 1u);
+          if (prev + 1u == args.n_segs) args.cand[atomicAdd(args.cand_count, 1u)] = m_idx;
         }
+      } else if (due != 0ull) {
+        // Partial minimum of slot s = 16 t + j: lanes j, j+16, j+32, j+48 of
+        // mn[t]; lane s picks its own tile's value.
+        double mine = 0.0;
+  #pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          int32_t r = min(mn[t], __shfl_xor(mn[t], 16, 64));
+          r = min(r, __shfl_xor(r, 32, 64));
+          double v = (double)__int_as_float(r);
+          if (ND == 3) v -= (double)cna[t];
+          if (kc == (uint32_t)t) mine = v;
+        }
+        // Stage the escalations in lanes [n_stg, n_stg + n_esc) (a forward
+        // permutation: due slots in order, the other lanes after them) and
+        // append 64 at a time, so the shared counter sees one atomic per 64.
+        const uint32_t n_esc = (uint32_t)__popcll(due);
+        if (n_stg + n_esc > (uint32_t)kWaveSize) flush();
+        const bool esc = (due >> lane) & 1ull;
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(due >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)due, 0u));
+        const uint32_t dest =
+            (esc ? n_stg + below : n_stg + n_esc + (lane - below)) & 63u;
+        const double ub = (mine > 0.0 ? mine : 0.0) + (double)args.err;
+        const int d4 = (int)(dest << 2);
+        const uint32_t p_idx = (uint32_t)__builtin_amdgcn_ds_permute(d4, (int)m_idx);
+        const float p_min = __int_as_float(__builtin_amdgcn_ds_permute(
+            d4, __float_as_int(__double2float_ru(ub))));
+        if (lane >= n_stg && lane < n_stg + n_esc) {
+          stg_idx = p_idx;
+          stg_min = p_min;
+        }
+        n_stg += n_esc;
+        if (n_stg == (uint32_t)kWaveSize) flush();
       }
-      const uint32_t took = (uint32_t)__popcll(free_m) < navail ? (uint32_t)__popcll(free_m) : navail;
-      const unsigned long long took_m = __ballot(take_s);
-      live |= took_m;
-      free_m &= ~took_m;
-      rhead += took;
-    }
-    if (live == 0ull) break;
-
-    // Two steps: 8 B tiles against the 4 A tiles (32 MFMAs).
-    const uint32_t s2 = next_step(next_step(step));
-    const uint32_t s3 = next_step(s2);
-    float bnx[8], onx[2][ND];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      bnx[q] = LoadTile(args.bt, (uint64_t)Uniform(s2) * 4u + q, lane4);
-      bnx[4 + q] = LoadTile(args.bt, (uint64_t)Uniform(s3) * 4u + q, lane4);
-    }
-    LoadStep<ND>(args.bx, args.by, args.bz, s2, lane, onx[0]);
-    LoadStep<ND>(args.bx, args.by, args.bz, s3, lane, onx[1]);
-    int32_t sm[4];  // this iteration's approximate minima
-    const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      sm[t] = (int32_t)kFltMaxBits;
-      // A tile without live slots (the cursor has drained) costs nothing.
-      if (((live >> (16 * t)) & 0xFFFFull) == 0ull) continue;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const floatx4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(bcur[q], bop[t], zero, 0, 0, 0);
-        sm[t] = Min3I(sm[t], __float_as_int(d[0]), __float_as_int(d[1]));
-        sm[t] = Min3I(sm[t], __float_as_int(d[2]), __float_as_int(d[3]));
+      live &= ~(hit | gone | due);
+  #pragma unroll
+      for (int q = 0; q < 8; ++q) bcur[q] = bnx[q];
+  #pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        ocur[0][d] = onx[0][d];
+        ocur[1][d] = onx[1][d];
       }
-      mn[t] = min(mn[t], sm[t]);
+      step = s2;
     }
-    gstep += 2;
-    evals += (unsigned long long)__popcll(live) * 128ull;
-
-    // Drops (proved below cmax2, or verified exactly) and escalations.
-    unsigned long long hit = 0, maybe = 0;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const float v = __int_as_float(sm[t]);
-      const float lim = ND == 3 ? thr + cna[t] : thr;
-      const float lim_hi = ND == 3 ? thr_hi + cna[t] : thr_hi;
-      unsigned long long c = __ballot(v < lim);
-      unsigned long long m = __ballot(v < lim_hi);
-      c |= c >> 32;
-      c |= c >> 16;
-      m |= m >> 32;
-      m |= m >> 16;
-      hit |= (c & 0xFFFFull) << (16 * t);
-      maybe |= (m & 0xFFFFull) << (16 * t);
-    }
-    hit &= live;
-    maybe &= live & ~hit;
-    while (maybe != 0ull) {
-      // Exact test of this iteration's 128 original B points (the
-      // reference's formula, strict '<').
-      const uint32_t sl = (uint32_t)__builtin_ctzll(maybe);
-      maybe &= maybe - 1ull;
-      const float ax = ReadLaneF(m_ax, sl), ay = ReadLaneF(m_ay, sl);
-      float d0, d1;
-      if constexpr (ND == 2) {
-        d0 = D2(ax, ay, ocur[0][0], ocur[0][1]);
-        d1 = D2(ax, ay, ocur[1][0], ocur[1][1]);
-      } else {
-        const float az = ReadLaneF(m_az, sl);
-        d0 = D2(ax, ay, az, ocur[0][0], ocur[0][1], ocur[0][2]);
-        d1 = D2(ax, ay, az, ocur[1][0], ocur[1][1], ocur[1][2]);
-      }
-      if (WaveAny(d0 < cmax2 || d1 < cmax2)) hit |= 1ull << sl;
-    }
-    unsigned long long due =
-        __ballot(gstep - m_start >= args.budget_steps) & live & ~hit;
-    if (due != 0ull) {
-      // Partial minimum of slot s = 16 t + j: lanes j, j+16, j+32, j+48 of
-      // mn[t]; lane s picks its own tile's value.
-      double mine = 0.0;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        int32_t r = min(mn[t], __shfl_xor(mn[t], 16, 64));
-        r = min(r, __shfl_xor(r, 32, 64));
-        double v = (double)__int_as_float(r);
-        if (ND == 3) v -= (double)cna[t];
-        if (kc == (uint32_t)t) mine = v;
-      }
-      // Stage the escalations in lanes [n_stg, n_stg + n_esc) (a forward
-      // permutation: due slots in order, the other lanes after them) and
-      // append 64 at a time, so the shared counter sees one atomic per 64.
-      const uint32_t n_esc = (uint32_t)__popcll(due);
-      if (n_stg + n_esc > (uint32_t)kWaveSize) flush();
-      const bool esc = (due >> lane) & 1ull;
-      const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-          (uint32_t)(due >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)due, 0u));
-      const uint32_t dest =
-          (esc ? n_stg + below : n_stg + n_esc + (lane - below)) & 63u;
-      const double ub = (mine > 0.0 ? mine : 0.0) + (double)args.err;
-      const int d4 = (int)(dest << 2);
-      const uint32_t p_idx = (uint32_t)__builtin_amdgcn_ds_permute(d4, (int)m_idx);
-      const float p_min = __int_as_float(__builtin_amdgcn_ds_permute(
-          d4, __float_as_int(__double2float_ru(ub))));
-      if (lane >= n_stg && lane < n_stg + n_esc) {
-        stg_idx = p_idx;
-        stg_min = p_min;
-      }
-      n_stg += n_esc;
-      if (n_stg == (uint32_t)kWaveSize) flush();
-    }
-    live &= ~(hit | due);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) bcur[q] = bnx[q];
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      ocur[0][d] = onx[0][d];
-      ocur[1][d] = onx[1][d];
-    }
-    step = s2;
   }
   if (n_stg > 0) flush();
   if (lane == 0) atomicAdd(args.evals, evals);
@@ -898,9 +952,10 @@ int LaunchSlots(hd_context* ctx, const float* a, int order_mode,
                 const float* bx, const float* by, const float* bz,
                 uint32_t n_steps, uint32_t budget_steps,
                 unsigned long long* key, unsigned long long* evals,
-                unsigned int* counters, uint32_t* esc_idx, float* esc_min) {
+                unsigned int* counters, uint32_t* esc_idx, float* esc_min,
+                bool reset_esc = true) {
   if (pos_begin >= pos_end) return HD_OK;
-  HD_TRY(FillU32(ctx, counters + 1, 0u, 1));
+  if (reset_esc) HD_TRY(FillU32(ctx, counters + 1, 0u, 1));
   ScanArgs args = {};
   args.a = a;
   args.order = order;
@@ -1003,10 +1058,11 @@ int LaunchMfma(hd_context* ctx, const float* a, int order_mode,
                const Feistel& perm_a, uint32_t pos_begin, uint32_t pos_end,
                const EbTarget& t, uint32_t budget_steps,
                unsigned long long* key, unsigned long long* evals,
-               unsigned int* counters, uint32_t* esc_idx, float* esc_min) {
+               unsigned int* counters, uint32_t* esc_idx, float* esc_min,
+               bool reset_esc = true) {
   if (pos_begin >= pos_end) return HD_OK;
   HD_TRY(FillU32(ctx, counters, pos_begin, 1));
-  HD_TRY(FillU32(ctx, counters + 1, 0u, 1));
+  if (reset_esc) HD_TRY(FillU32(ctx, counters + 1, 0u, 1));
   MfmaArgs args = {};
   args.a = a;
   args.order = order;
@@ -1014,7 +1070,9 @@ int LaunchMfma(hd_context* ctx, const float* a, int order_mode,
   args.perm_a = perm_a;
   args.order_mode = order_mode;
   args.pos_end = pos_end;
-  args.cursor = counters;
+  args.cursors = counters;
+  args.seg_steps = t.n_steps;
+  args.n_segs = 1;
   args.bt = t.bt;
   args.bx = t.bx;
   args.by = t.by;
@@ -1033,7 +1091,9 @@ int LaunchMfma(hd_context* ctx, const float* a, int order_mode,
   args.evals = evals;
   const uint32_t n_entries = pos_end - pos_begin;
   int blocks = MfmaResidentBlocks<ND>(ctx);
-  const uint32_t waves_needed = (n_entries + 63u) / 64u;
+  // At least 8 entries per slot: with fewer, most of the round is the
+  // drain, where tiles with a few live slots still cost all their MFMAs.
+  const uint32_t waves_needed = (n_entries + 511u) / 512u;
   if ((uint32_t)blocks * 4u > waves_needed)
     blocks = (int)std::max(1u, (waves_needed + 3u) / 4u);
   const uint32_t n_waves = (uint32_t)blocks * (kBlock / kWaveSize);
@@ -1045,10 +1105,69 @@ int LaunchMfma(hd_context* ctx, const float* a, int order_mode,
 }
 
 // Reads a device counter and, optionally, the current cmax2 (one sync).
+// Segmented filter scan of `list` (the tail): B split into segments that
+// different waves scan, each (entry, segment) a slot's item; an entry is
+// dropped as soon as any segment finds a hit, and the entries that complete
+// every segment without one (exact minimum >= cmax2: candidates for the
+// maximum) are written to cand / *cand_count for an exact scan.
+template <int ND>
+int LaunchMfmaSplit(hd_context* ctx, const float* a, const uint32_t* list,
+                    const float* list_min, uint32_t n_list, const EbTarget& t,
+                    unsigned long long* key, unsigned long long* evals,
+                    uint32_t* cand, unsigned int* cand_count) {
+  HD_TRY(FillU32(ctx, cand_count, 0u, 1));
+  if (n_list == 0) return HD_OK;
+  const int blocks = MfmaResidentBlocks<ND>(ctx);
+  const uint32_t n_waves = (uint32_t)blocks * (kBlock / kWaveSize);
+  // Segments of about 1024 steps (at least 16 of them): long items leave
+  // the end of the launch to a few waves, short ones multiply the work
+  // spent after a hit until the other segments see it.
+  uint32_t n_segs = std::max(16u, t.n_steps / 1024u);
+  n_segs = std::min(n_segs, std::max(1u, t.n_steps / 256u));
+  const uint32_t seg_steps = (t.n_steps + n_segs - 1) / n_segs;
+  n_segs = (t.n_steps + seg_steps - 1) / seg_steps;
+  MfmaArgs args = {};
+  args.a = a;
+  args.order = list;
+  args.list_min = list_min;
+  args.perm_a = MakeFeistel(1, 0, 0);
+  args.order_mode = kOrderArray;
+  args.pos_end = n_list;
+  HD_TRY(ContextArray(ctx, "eb_mfma_cursors", n_segs, &args.cursors));
+  HD_TRY(FillU32(ctx, args.cursors, 0u, n_segs));
+  args.seg_steps = seg_steps;
+  args.n_segs = n_segs;
+  HD_TRY(ContextArray(ctx, "eb_broken", n_list, &args.broken));
+  HD_TRY(ContextArray(ctx, "eb_gdone", n_list, &args.gdone));
+  HD_TRY(FillU32(ctx, args.broken, 0u, n_list));
+  HD_TRY(FillU32(ctx, args.gdone, 0u, n_list));
+  args.cand = cand;
+  args.cand_count = cand_count;
+  args.bt = t.bt;
+  args.bx = t.bx;
+  args.by = t.by;
+  args.bz = t.bz;
+  args.n_steps = t.n_steps;
+  args.budget_steps = 0xFFFFFFFFu;
+  args.cx = t.center[0];
+  args.cy = t.center[1];
+  args.cz = t.center[2];
+  args.err = t.err;
+  args.off = t.off;
+  args.key = key;
+  args.evals = evals;
+  const uint32_t per_seg = std::max(1u, n_waves / n_segs);
+  args.claim = std::min<uint32_t>(
+      kWaveSize, std::max<uint32_t>(16u, (n_list + per_seg - 1) / per_seg));
+  EbMfmaKernel<ND><<<blocks, kBlock, 0, ctx->stream>>>(args);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
+
 int ReadCount(hd_context* ctx, const unsigned int* d_count, uint32_t* out,
               const unsigned long long* key = nullptr, float* cmax2 = nullptr) {
   void* pinned = nullptr;
-  HD_TRY(ContextPinned(ctx, 64, &pinned));
+  HD_TRY(ContextPinned(ctx, 256, &pinned));
   unsigned int* h = static_cast<unsigned int*>(pinned);
   if (d_count != nullptr)
     HD_HIP_TRY(hipMemcpyAsync(h, d_count, 4, hipMemcpyDeviceToHost,
@@ -1064,6 +1183,14 @@ int ReadCount(hd_context* ctx, const unsigned int* d_count, uint32_t* out,
     *cmax2 = f;
   }
   return HD_OK;
+}
+
+// Profiling stage of budgeted round r (0-based) on the exact scan or the
+// matrix-core filter.
+const char* RoundStage(int r, bool filt) {
+  static const char* kScan[] = {"eb_scan_r1", "eb_scan_r2", "eb_scan_r3", "eb_scan_r4"};
+  static const char* kMfma[] = {"eb_mfma_r1", "eb_mfma_r2", "eb_mfma_r3", "eb_mfma_r4"};
+  return filt ? kMfma[r] : kScan[r];
 }
 
 // The filter pays off while steps that need an exact test stay rare, i.e.
@@ -1103,8 +1230,10 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
                 uint32_t begin, uint32_t end, uint32_t first_budget,
                 unsigned long long* key, unsigned long long* ev,
                 EbRoundStats* rs) {
-  static const uint32_t kBudgets[] = {16u, 256u, 4096u};
-  const int n_rounds = first_budget ? 1 : 3;
+  static const uint32_t kBudgets[] = {16u, 256u, 1024u};
+  constexpr int kRounds = (int)(sizeof(kBudgets) / sizeof(kBudgets[0]));
+  static_assert(kRounds <= kMaxRounds, "round statistics");
+  const int n_rounds = first_budget ? 1 : kRounds;
   unsigned int* counters = nullptr;
   HD_TRY(ContextArray(ctx, "eb_counters", 4, &counters));
   uint32_t *idx_in = nullptr, *idx_out = nullptr, *idx_sorted = nullptr;
@@ -1117,31 +1246,54 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
   const uint32_t* list = order;
   const float* list_min = order_min;
   float cmax2 = 0.f;
-  for (int r = 0; r < n_rounds && begin < end; ++r) {
+  // A pseudo-random order of a large set ranks on a sample: the first round
+  // (short budget, cmax2 still far from its final value, exact scan) only
+  // visits the first eighth of the order, enough for its top completions
+  // to lift cmax2; the rest of the order joins the second round, which can
+  // then run on the matrix-core filter.
+  uint32_t extra_begin = 0, extra_end = 0;
+  if (!first_budget && order_mode != kOrderArray && end - begin >= (1u << 21)) {
+    extra_begin = begin + std::max<uint32_t>(65536u, (end - begin) / 8u);
+    extra_end = end;
+    end = extra_begin;
+  }
+  for (int r = 0; r < n_rounds && (begin < end || extra_begin < extra_end);
+       ++r) {
     const uint32_t budget = first_budget ? first_budget : kBudgets[r];
-    rs->round_points[r] += end - begin;
+    const bool with_extra = r == 1 && extra_begin < extra_end;
+    rs->round_points[r] +=
+        (end - begin) + (with_extra ? extra_end - extra_begin : 0u);
     if (t.bt != nullptr && budget < t.n_steps)
       HD_TRY(ReadCount(ctx, nullptr, nullptr, key, &cmax2));
     // Matrix-core filter when its threshold keeps most of cmax2, else the
     // exact scan.
     const bool filt = UseFilter(t, budget, cmax2);
     if (filt) rs->mfma_rounds |= 1u << r;
-    const int st_slot = StageBegin(
-        ctx, filt ? (r == 0 ? "eb_mfma" : "eb_mfma_more")
-                  : (r == 0 ? "eb_slot" : "eb_slot_more"));
-    if (filt)
-      HD_TRY(LaunchMfma<ND>(ctx, a, mode, list, list_min, perm_a, begin, end, t,
-                            budget, key, ev + r, counters, idx_out, min_out));
-    else
-      HD_TRY(LaunchSlots<ND>(ctx, a, mode, list, list_min, perm_a, begin, end,
-                             t.bx, t.by, t.bz, t.n_steps, budget, key, ev + r,
-                             counters, idx_out, min_out));
+    const int st_slot = StageBegin(ctx, RoundStage(r, filt));
+    auto launch = [&](int m, const uint32_t* l, const float* lm, uint32_t b,
+                      uint32_t e, bool reset) -> int {
+      if (filt)
+        return LaunchMfma<ND>(ctx, a, m, l, lm, perm_a, b, e, t, budget, key,
+                              ev + r, counters, idx_out, min_out, reset);
+      return LaunchSlots<ND>(ctx, a, m, l, lm, perm_a, b, e, t.bx, t.by, t.bz,
+                             t.n_steps, budget, key, ev + r, counters, idx_out,
+                             min_out, reset);
+    };
+    bool reset = true;
+    if (begin < end) {
+      HD_TRY(launch(mode, list, list_min, begin, end, true));
+      reset = false;
+    }
+    if (with_extra) {
+      HD_TRY(launch(order_mode, order, nullptr, extra_begin, extra_end, reset));
+      extra_begin = extra_end;
+    }
     StageEnd(ctx, st_slot);
     uint32_t n_esc = 0;
     HD_TRY(ReadCount(ctx, counters + 1, &n_esc));
     if (n_esc == 0) {
       begin = end = 0;
-      break;
+      continue;
     }
     const int st_sort = StageBegin(ctx, "eb_sort");
     HD_TRY(SortPairsDescF32(ctx, min_out, min_sorted, idx_out, idx_sorted,
@@ -1151,7 +1303,7 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
     rs->top_points += k_top;
     const int st_top = StageBegin(ctx, "eb_top");
     HD_TRY((LaunchSplit<ND>(ctx, a, idx_sorted, min_sorted, k_top, t.bx, t.by,
-                           t.bz, t.n_steps, key, ev + 3)));
+                           t.bz, t.n_steps, key, ev + kMaxRounds)));
     StageEnd(ctx, st_top);
     // The sorted escalations become the next round's list; the next round
     // writes its sort into the buffer this round read from.
@@ -1168,14 +1320,42 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
     }
   }
   if (begin < end) {
-    // What is left needed more than the last budget: cooperative scans,
-    // each group split over at least 16 waves.
+    // What is left needed more than the last budget: segmented scans, each
+    // entry spread over at least 16 waves. With the filter, only the
+    // entries that no segment can drop get an exact scan afterwards.
     rs->tail_points += end - begin;
-    const int st_tail = StageBegin(ctx, "eb_tail");
-    HD_TRY((LaunchSplit<ND>(ctx, a, list + begin,
-                           list_min ? list_min + begin : nullptr, end - begin,
-                           t.bx, t.by, t.bz, t.n_steps, key, ev + 4, 16u)));
-    StageEnd(ctx, st_tail);
+    const uint32_t n_tail = end - begin;
+    const float* tail_min = list_min ? list_min + begin : nullptr;
+    bool filt = false;
+    if (t.bt != nullptr && t.n_steps >= 4096u) {
+      HD_TRY(ReadCount(ctx, nullptr, nullptr, key, &cmax2));
+      filt = t.err < 0.5f * cmax2;
+    }
+    rs->tail_filtered = filt;
+    if (filt) {
+      uint32_t* cand = nullptr;
+      unsigned int* cand_count = nullptr;
+      HD_TRY(ContextArray(ctx, "eb_cand", n_tail, &cand));
+      HD_TRY(ContextArray(ctx, "eb_cand_count", 1, &cand_count));
+      const int st_tail = StageBegin(ctx, "eb_mfma_tail");
+      HD_TRY((LaunchMfmaSplit<ND>(ctx, a, list + begin, tail_min, n_tail, t,
+                                  key, ev + kMaxRounds + 1, cand,
+                                  cand_count)));
+      StageEnd(ctx, st_tail);
+      uint32_t n_cand = 0;
+      HD_TRY(ReadCount(ctx, cand_count, &n_cand));
+      rs->exact_points += n_cand;
+      const int st_exact = StageBegin(ctx, "eb_tail_exact");
+      HD_TRY((LaunchSplit<ND>(ctx, a, cand, nullptr, n_cand, t.bx, t.by, t.bz,
+                             t.n_steps, key, ev + kMaxRounds + 2, 16u)));
+      StageEnd(ctx, st_exact);
+    } else {
+      const int st_tail = StageBegin(ctx, "eb_tail");
+      HD_TRY((LaunchSplit<ND>(ctx, a, list + begin, tail_min, n_tail, t.bx,
+                             t.by, t.bz, t.n_steps, key, ev + kMaxRounds + 1,
+                             16u)));
+      StageEnd(ctx, st_tail);
+    }
   }
   return HD_OK;
 }
@@ -1200,12 +1380,12 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
     HD_TRY(LearnedOrder(ctx, in, &order));
     order_mode = kOrderArray;
   }
-  // key, then one evaluation counter per stage.
+  // key, then one evaluation counter per stage: seed, the rounds, top
+  // completions, tail.
   unsigned long long* key = nullptr;
-  HD_TRY(ContextArray(ctx, "eb_key", 8, &key));
-  // counters: seed, rounds 1-3, top completions, tail
+  HD_TRY(ContextArray(ctx, "eb_key", 2 + kEbCounters, &key));
   unsigned long long* ev = key + 1;
-  HD_TRY(FillU64(ctx, key, 0ull, 8));
+  HD_TRY(FillU64(ctx, key, 0ull, 2 + kEbCounters));
   StageEnd(ctx, st_prep);
 
   // Stage 1: exact minima of the first s0 points of the order.
@@ -1231,23 +1411,29 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
                          s0, n_a, in.slot_budget_steps, key, ev + 1, &rs));
 
   const int st_fin = StageBegin(ctx, "eb_finish");
-  HD_TRY(FinishDirected(ctx, ND, in.a, in.b, in.n_b, key, ev, 6, out));
+  HD_TRY(FinishDirected(ctx, ND, in.a, in.b, in.n_b, key, ev, 1 + kEbCounters,
+                        out));
   StageEnd(ctx, st_fin);
   if (ctx->profiling) {
     void* pinned = nullptr;
-    HD_TRY(ContextPinned(ctx, 64, &pinned));
+    HD_TRY(ContextPinned(ctx, 256, &pinned));
     unsigned long long* h = static_cast<unsigned long long*>(pinned);
-    HD_HIP_TRY(hipMemcpyAsync(h, ev, 48, hipMemcpyDeviceToHost, ctx->stream));
+    HD_HIP_TRY(hipMemcpyAsync(h, ev, 8 * (1 + kEbCounters), hipMemcpyDeviceToHost,
+                              ctx->stream));
     HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
     StageCount(ctx, "eb_seed", h[0], s0);
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 0; r < kMaxRounds; ++r) {
+      if (rs.round_points[r] == 0) continue;
       const bool filt = (rs.mfma_rounds >> r) & 1u;
-      StageCount(ctx, filt ? (r == 0 ? "eb_mfma" : "eb_mfma_more")
-                           : (r == 0 ? "eb_slot" : "eb_slot_more"),
-                 h[1 + r], rs.round_points[r]);
+      StageCount(ctx, RoundStage(r, filt), h[1 + r], rs.round_points[r]);
     }
-    StageCount(ctx, "eb_top", h[4], rs.top_points);
-    StageCount(ctx, "eb_tail", h[5], rs.tail_points);
+    StageCount(ctx, "eb_top", h[1 + kMaxRounds], rs.top_points);
+    if (rs.tail_filtered) {
+      StageCount(ctx, "eb_mfma_tail", h[2 + kMaxRounds], rs.tail_points);
+      StageCount(ctx, "eb_tail_exact", h[3 + kMaxRounds], rs.exact_points);
+    } else {
+      StageCount(ctx, "eb_tail", h[2 + kMaxRounds], rs.tail_points);
+    }
   }
   return HD_OK;
 }

@@ -486,9 +486,12 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
   HD_TRY(BuildGrid(ctx, ND, in.b, in.n_b, in.n_points_cell, &grid));
   StageEnd(ctx, st_build);
 
+  // key, seed and query counters, then the early-break counters of the
+  // escalated queries.
+  constexpr int kKeys = 3 + kEbCounters;
   unsigned long long* key = nullptr;
-  HD_TRY(ContextArray(ctx, "grid_key", 8, &key));
-  HD_TRY(FillU64(ctx, key, 0ull, 8));
+  HD_TRY(ContextArray(ctx, "grid_key", kKeys, &key));
+  HD_TRY(FillU64(ctx, key, 0ull, kKeys));
   unsigned int* esc_count = nullptr;
   HD_TRY(ContextArray(ctx, "grid_esc_count", 1, &esc_count));
   HD_TRY(FillU32(ctx, esc_count, 0u, 1));
@@ -543,7 +546,7 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
     // Queries that needed too many cells: exact early-break scans, starting
     // from the partial minimum the grid found.
     void* pinned = nullptr;
-    HD_TRY(ContextPinned(ctx, 64, &pinned));
+    HD_TRY(ContextPinned(ctx, 256, &pinned));
     unsigned int* h = static_cast<unsigned int*>(pinned);
     HD_HIP_TRY(hipMemcpyAsync(h, esc_count, 4, hipMemcpyDeviceToHost,
                               ctx->stream));
@@ -561,19 +564,22 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
     }
   }
   const int st_fin = StageBegin(ctx, "grid_finish");
-  HD_TRY(FinishDirected(ctx, ND, in.a, in.b, in.n_b, key, key + 1, 7, out));
+  HD_TRY(FinishDirected(ctx, ND, in.a, in.b, in.n_b, key, key + 1, kKeys - 1,
+                        out));
   StageEnd(ctx, st_fin);
   if (ctx->profiling) {
     void* pinned = nullptr;
-    HD_TRY(ContextPinned(ctx, 64, &pinned));
+    HD_TRY(ContextPinned(ctx, 256, &pinned));
     unsigned long long* h = static_cast<unsigned long long*>(pinned);
-    HD_HIP_TRY(hipMemcpyAsync(h, key + 1, 56, hipMemcpyDeviceToHost,
+    HD_HIP_TRY(hipMemcpyAsync(h, key + 1, 8 * (kKeys - 1), hipMemcpyDeviceToHost,
                               ctx->stream));
     HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
     StageCount(ctx, "grid_seed", h[0], s0);
     StageCount(ctx, "grid_query", h[1], n_a - s0);
     StageCount(ctx, "grid_build", 0, in.n_b);
-    StageCount(ctx, "hybrid_eb", h[2] + h[3] + h[4] + h[5] + h[6], n_esc);
+    unsigned long long eb_evals = 0;
+    for (int i = 2; i < kKeys - 1; ++i) eb_evals += h[i];
+    StageCount(ctx, "hybrid_eb", eb_evals, n_esc);
   }
   return HD_OK;
 }

@@ -195,17 +195,24 @@ struct EbTarget {
   float err = 0.f;
   float off = 0.f;  // 3D: C, which keeps the approximate values >= -err
 };
+// Budgeted rounds of the early-break pipeline, and the evaluation counters
+// a directed run keeps for them: one per round, then the top completions,
+// the tail and the tail's exact follow-up.
+constexpr int kMaxRounds = 4;
+constexpr int kEbCounters = kMaxRounds + 3;
 struct EbRoundStats {
-  uint64_t round_points[3] = {0, 0, 0};
+  uint64_t round_points[kMaxRounds] = {0, 0, 0, 0};
   uint32_t mfma_rounds = 0;
   uint64_t top_points = 0;
   uint64_t tail_points = 0;
+  uint64_t exact_points = 0;    // tail entries no segment could drop
+  bool tail_filtered = false;
 };
 int EbPrepareTarget(hd_context* ctx, int n_dims, const float* b, uint32_t n_b,
                     bool shuffle, int32_t seed, EbTarget* t);
 // Finishes the listed A points (indices, with optional partial minima over
-// a subset of B) exactly with early break, raising *key; ev[0..4] count
-// evaluations of the rounds, the top completions and the tail.
+// a subset of B) exactly with early break, raising *key; ev[0..kEbCounters)
+// count evaluations of the rounds, the top completions and the tail.
 int EbFinishList(hd_context* ctx, int n_dims, const float* a, uint32_t n_a,
                  const EbTarget& t, const uint32_t* list, const float* list_min,
                  uint32_t n_list, unsigned long long* key,
