@@ -42,11 +42,12 @@ MFMA_F32_TFLOPS = 157.3
 # 8 FLOP per A/B pair).
 EB_KERNELS = {
     "EbScanKernel": {"full": "EbScanKernel<2,8,4>",
-                     "stages": ("eb_seed", "eb_scan_r1", "eb_scan_r2", "eb_scan_r3",
-                                "eb_scan_r4", "eb_top", "eb_tail", "eb_tail_exact")},
+                     "stages": ("eb_seed", "eb_scan_r1", "eb_scan_r2", "eb_scan_rest",
+                                "eb_scan_r3", "eb_scan_r4", "eb_top", "eb_tail",
+                                "eb_tail_exact")},
     "EbMfmaKernel": {"full": "EbMfmaKernel<2>",
-                     "stages": ("eb_mfma_r1", "eb_mfma_r2", "eb_mfma_r3", "eb_mfma_r4",
-                                "eb_mfma_tail")},
+                     "stages": ("eb_mfma_r1", "eb_mfma_r2", "eb_mfma_rest", "eb_mfma_r3",
+                                "eb_mfma_r4", "eb_mfma_tail")},
 }
 MFMA_FLOP_PER_PAIR = 8
 

@@ -1229,7 +1229,7 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
                 const float* order_min, const Feistel& perm_a,
                 uint32_t begin, uint32_t end, uint32_t first_budget,
                 unsigned long long* key, unsigned long long* ev,
-                EbRoundStats* rs) {
+                EbRoundStats* rs, int first_round = 0) {
   static const uint32_t kBudgets[] = {16u, 256u, 1024u};
   constexpr int kRounds = (int)(sizeof(kBudgets) / sizeof(kBudgets[0]));
   static_assert(kRounds <= kMaxRounds, "round statistics");
@@ -1257,12 +1257,11 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
     extra_end = end;
     end = extra_begin;
   }
-  for (int r = 0; r < n_rounds && (begin < end || extra_begin < extra_end);
-       ++r) {
+  for (int r = first_budget ? 0 : first_round;
+       r < n_rounds && (begin < end || extra_begin < extra_end); ++r) {
     const uint32_t budget = first_budget ? first_budget : kBudgets[r];
     const bool with_extra = r == 1 && extra_begin < extra_end;
-    rs->round_points[r] +=
-        (end - begin) + (with_extra ? extra_end - extra_begin : 0u);
+    rs->round_points[r] += end - begin;
     if (t.bt != nullptr && budget < t.n_steps)
       HD_TRY(ReadCount(ctx, nullptr, nullptr, key, &cmax2));
     // Matrix-core filter when its threshold keeps most of cmax2, else the
@@ -1271,24 +1270,31 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
     if (filt) rs->mfma_rounds |= 1u << r;
     const int st_slot = StageBegin(ctx, RoundStage(r, filt));
     auto launch = [&](int m, const uint32_t* l, const float* lm, uint32_t b,
-                      uint32_t e, bool reset) -> int {
+                      uint32_t e, bool reset, unsigned long long* evc) -> int {
       if (filt)
         return LaunchMfma<ND>(ctx, a, m, l, lm, perm_a, b, e, t, budget, key,
-                              ev + r, counters, idx_out, min_out, reset);
+                              evc, counters, idx_out, min_out, reset);
       return LaunchSlots<ND>(ctx, a, m, l, lm, perm_a, b, e, t.bx, t.by, t.bz,
-                             t.n_steps, budget, key, ev + r, counters, idx_out,
+                             t.n_steps, budget, key, evc, counters, idx_out,
                              min_out, reset);
     };
     bool reset = true;
     if (begin < end) {
-      HD_TRY(launch(mode, list, list_min, begin, end, true));
+      HD_TRY(launch(mode, list, list_min, begin, end, true, ev + r));
       reset = false;
     }
+    StageEnd(ctx, st_slot);
     if (with_extra) {
-      HD_TRY(launch(order_mode, order, nullptr, extra_begin, extra_end, reset));
+      // The part of the order the first round did not visit (a stage of
+      // its own: one kernel launch per stage call).
+      const int st_x = StageBegin(ctx, filt ? "eb_mfma_rest" : "eb_scan_rest");
+      HD_TRY(launch(order_mode, order, nullptr, extra_begin, extra_end, reset,
+                    ev + kMaxRounds + 3));
+      StageEnd(ctx, st_x);
+      rs->rest_points += extra_end - extra_begin;
+      rs->rest_filtered = filt;
       extra_begin = extra_end;
     }
-    StageEnd(ctx, st_slot);
     uint32_t n_esc = 0;
     HD_TRY(ReadCount(ctx, counters + 1, &n_esc));
     if (n_esc == 0) {
@@ -1407,8 +1413,12 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
   StageEnd(ctx, st_seed);
 
   EbRoundStats rs;
+  // The first round ranks points for the top completions; a learned order
+  // has ranked them already (its seed usually reaches the final cmax2), so
+  // it starts with the second.
   HD_TRY(ProcessOpen<ND>(ctx, in.a, n_a, t, order_mode, order, nullptr, perm_a,
-                         s0, n_a, in.slot_budget_steps, key, ev + 1, &rs));
+                         s0, n_a, in.slot_budget_steps, key, ev + 1, &rs,
+                         order_mode == kOrderArray ? 1 : 0));
 
   const int st_fin = StageBegin(ctx, "eb_finish");
   HD_TRY(FinishDirected(ctx, ND, in.a, in.b, in.n_b, key, ev, 1 + kEbCounters,
@@ -1428,6 +1438,9 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
       StageCount(ctx, RoundStage(r, filt), h[1 + r], rs.round_points[r]);
     }
     StageCount(ctx, "eb_top", h[1 + kMaxRounds], rs.top_points);
+    if (rs.rest_points != 0)
+      StageCount(ctx, rs.rest_filtered ? "eb_mfma_rest" : "eb_scan_rest",
+                 h[4 + kMaxRounds], rs.rest_points);
     if (rs.tail_filtered) {
       StageCount(ctx, "eb_mfma_tail", h[2 + kMaxRounds], rs.tail_points);
       StageCount(ctx, "eb_tail_exact", h[3 + kMaxRounds], rs.exact_points);

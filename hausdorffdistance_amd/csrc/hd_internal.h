@@ -197,9 +197,10 @@ struct EbTarget {
 };
 // Budgeted rounds of the early-break pipeline, and the evaluation counters
 // a directed run keeps for them: one per round, then the top completions,
-// the tail and the tail's exact follow-up.
+// the tail, the tail's exact follow-up and the part of the order a sampled
+// first round left for the second.
 constexpr int kMaxRounds = 4;
-constexpr int kEbCounters = kMaxRounds + 3;
+constexpr int kEbCounters = kMaxRounds + 4;
 struct EbRoundStats {
   uint64_t round_points[kMaxRounds] = {0, 0, 0, 0};
   uint32_t mfma_rounds = 0;
@@ -207,6 +208,8 @@ struct EbRoundStats {
   uint64_t tail_points = 0;
   uint64_t exact_points = 0;    // tail entries no segment could drop
   bool tail_filtered = false;
+  uint64_t rest_points = 0;     // order positions the sampled first round left
+  bool rest_filtered = false;
 };
 int EbPrepareTarget(hd_context* ctx, int n_dims, const float* b, uint32_t n_b,
                     bool shuffle, int32_t seed, EbTarget* t);
