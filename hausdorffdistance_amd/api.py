@@ -384,6 +384,16 @@ class Model:
                                          None, None))
         return out
 
+    def normalisation(self) -> tuple[np.ndarray, np.ndarray]:
+        """Input scaling of the network: x = (point - lower) * inv_extent."""
+        lib = _lib.load()
+        lo = np.zeros(3, np.float32)
+        inv = np.zeros(3, np.float32)
+        _lib.check(lib.hd_mlp_get_params(self._h, None,
+                                         lo.ctypes.data_as(C.POINTER(C.c_float)),
+                                         inv.ctypes.data_as(C.POINTER(C.c_float))))
+        return lo[: self.n_dims], inv[: self.n_dims]
+
     def close(self) -> None:
         if self._h is not None:
             _lib.load().hd_mlp_destroy(self._h)

@@ -11,6 +11,8 @@ namespace hd {
 
 int SortPairsDescF32(hd_context* ctx, const float* k_in, float* k_out,
                      const uint32_t* v_in, uint32_t* v_out, uint32_t n);
+int SortPairsU16(hd_context* ctx, const uint16_t* k_in, uint16_t* k_out,
+                 const uint32_t* v_in, uint32_t* v_out, uint32_t n);
 int SortPairsU32(hd_context* ctx, const uint32_t* k_in, uint32_t* k_out,
                  const uint32_t* v_in, uint32_t* v_out, uint32_t n,
                  int end_bit);

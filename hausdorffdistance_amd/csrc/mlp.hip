@@ -17,8 +17,9 @@
 //
 // Kernels: forward+backward with one lane per sample (weights in LDS as
 // FP32 converted from BF16), a per-parameter kernel that reduces the
-// gradient over the batch and applies Adam, and a one-lane-per-point
-// inference kernel.
+// gradient over the batch and applies Adam, and an inference kernel that
+// runs the 64x64 layer on matrix cores (BF16 operands, FP32 accumulation;
+// the sine activations are rounded to BF16 as MFMA operands).
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
@@ -136,20 +137,6 @@ HD_DEV float Layer2Unit(const float* s_w, int w2, int b2, int j,
 #pragma unroll
   for (int k = 0; k < kH; ++k) acc = fmaf(row[k], h1[k], acc);
   return acc;
-}
-
-template <int ND>
-HD_DEV float Forward(const float* s_w, const float* x) {
-  const Layout L = MakeLayout(ND);
-  float h1[kH];
-  Layer1<ND>(s_w, x, h1);
-  float out = s_w[L.b3];
-#pragma unroll 1
-  for (int j = 0; j < kH; ++j) {
-    const float h2 = fmaxf(Layer2Unit(s_w, L.w2, L.b2, j, h1), 0.f);
-    out = fmaf(s_w[L.w3 + j], h2, out);
-  }
-  return out;
 }
 
 struct TrainArgs {
@@ -314,23 +301,115 @@ __global__ void MlpPrepKernel(const float* __restrict__ a,
   }
 }
 
-template <int ND>
+typedef short MlpShortx4 __attribute__((ext_vector_type(4)));
+typedef float MlpFloatx4 __attribute__((ext_vector_type(4)));
+
+// BF16 bits of two floats (round to nearest even), low half first.
+HD_DEV uint32_t PackBf16(float lo, float hi) {
+  return (uint32_t)F32ToBf16(lo) | ((uint32_t)F32ToBf16(hi) << 16);
+}
+
+// What inference writes: predictions, or 16-bit sort keys (ascending key =
+// descending prediction, 2^-11 steps over [-16, 16)) with point indices.
+struct PredictOut {
+  float* pred;
+  uint16_t* key;
+  uint32_t* idx;
+};
+
+// Inference on matrix cores, 16 points per wave step. Lane l works on point
+// l % 16 of the step and hidden units 16 kb + 4 (l / 16) + r (kb, r in
+// 0..3): it evaluates those 16 sines, rounds them to BF16 and holds them as
+// the B operand of v_mfma_f32_16x16x16_bf16 (B[k][j] at lane j + 16 (k / 4),
+// element k % 4). The A operand is a 16x16 block of W2 (A[i][k] at lane
+// i + 16 (k / 4)); the accumulator starts at b2 and ends as
+// H2[16 ib + 4 (l / 16) + r][point l % 16]. The output layer is a dot
+// product over each lane's 16 units (FP32) and a sum over the four lane
+// groups. All weights stay in registers for the wave's lifetime.
+template <int ND, bool kKeys>
 __global__ __launch_bounds__(256) void MlpPredictKernel(
     const uint16_t* __restrict__ wbf, const float* __restrict__ pts,
     uint64_t n, float lo0, float lo1, float lo2, float inv0, float inv1,
-    float inv2, float* __restrict__ out) {
-  __shared__ float s_w[kH * ND + kH + kH * kH + kH + kH + 1];
-  LoadWeights<ND>(wbf, s_w);
+    float inv2, PredictOut o) {
+  const Layout L = MakeLayout(ND);
+  const uint32_t lane = threadIdx.x & 63u, col = lane & 15u, grp = lane >> 4;
   const float lo[3] = {lo0, lo1, lo2};
   const float inv[3] = {inv0, inv1, inv2};
-  // One point per thread: a loop over points would let the compiler hoist
-  // every weight out of LDS into registers and collapse occupancy.
-  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (i < n) {
+  // v_sin takes revolutions: the first layer is scaled by 1 / (2 pi).
+  const float kRev = 0.15915494309189535f;
+  float w1[16][ND], b1[16], b2[16], w3[16];
+  MlpShortx4 w2[4][4];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int u = 16 * (q >> 2) + 4 * (int)grp + (q & 3);
+    b1[q] = Bf16ToF32(wbf[L.b1 + u]) * kRev;
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+      w1[q][d] = Bf16ToF32(wbf[L.w1 + u * ND + d]) * kRev;
+    b2[q] = Bf16ToF32(wbf[L.b2 + u]);
+    w3[q] = Bf16ToF32(wbf[L.w3 + u]);
+  }
+#pragma unroll
+  for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        w2[ib][kb][r] = (short)wbf[L.w2 + (16 * ib + (int)col) * kH + 16 * kb +
+                                   4 * (int)grp + r];
+  const float b3 = Bf16ToF32(wbf[L.b3]);
+
+  const uint64_t n_tiles = (n + 15) / 16;
+  const uint64_t n_waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  for (uint64_t tile = (uint64_t)blockIdx.x * (blockDim.x >> 6) +
+                       (threadIdx.x >> 6);
+       tile < n_tiles; tile += n_waves) {
+    const uint64_t p = tile * 16 + col;
+    const uint64_t pc = p < n ? p : n - 1;
     float x[ND];
 #pragma unroll
-    for (int k = 0; k < ND; ++k) x[k] = (pts[i * ND + k] - lo[k]) * inv[k];
-    out[i] = Forward<ND>(s_w, x);
+    for (int d = 0; d < ND; ++d) x[d] = (pts[pc * ND + d] - lo[d]) * inv[d];
+    MlpShortx4 h1[4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      float h[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float z = b1[4 * kb + r];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) z = fmaf(w1[4 * kb + r][d], x[d], z);
+        h[r] = __builtin_amdgcn_sinf(z);
+      }
+      const uint32_t p01 = PackBf16(h[0], h[1]), p23 = PackBf16(h[2], h[3]);
+      h1[kb][0] = (short)(p01 & 0xFFFFu);
+      h1[kb][1] = (short)(p01 >> 16);
+      h1[kb][2] = (short)(p23 & 0xFFFFu);
+      h1[kb][3] = (short)(p23 >> 16);
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib) {
+      MlpFloatx4 acc = {b2[4 * ib], b2[4 * ib + 1], b2[4 * ib + 2],
+                        b2[4 * ib + 3]};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[ib][kb], h1[kb],
+                                                         acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) s = fmaf(w3[4 * ib + r], fmaxf(acc[r], 0.f), s);
+    }
+    s += __shfl_xor(s, 16);
+    s += __shfl_xor(s, 32);
+    s += b3;
+    if (grp == 0 && p < n) {
+      if (kKeys) {
+        const float q = fminf(fmaxf((s + 16.f) * 2048.f, 0.f), 65535.f);
+        o.key[p] = (uint16_t)(65535u - (uint32_t)q);
+        o.idx[p] = (uint32_t)p;
+      } else {
+        o.pred[p] = s;
+      }
+    }
   }
 }
 
@@ -553,21 +632,25 @@ int TrainImpl(hd_context* ctx, const float* d_a, uint64_t n_a,
 
 }  // namespace
 
-int PredictInto(hd_context* ctx, const hd_mlp* m, const float* pts, uint64_t n,
-                float* out) {
+// Enough waves to fill the chip; each wave strides over 16-point steps.
+template <int ND, bool kKeys>
+int LaunchPredict(hd_context* ctx, const hd_mlp* m, const float* pts,
+                  uint64_t n, PredictOut o) {
   if (n == 0) return HD_OK;
-  if ((n + 255) / 256 > 0x7FFFFFFFull) return Fail(HD_EINVAL, "too many points");
-  const int blocks = (int)((n + 255) / 256);
-  if (m->n_dims == 2)
-    MlpPredictKernel<2><<<std::max(blocks, 1), 256, 0, ctx->stream>>>(
-        m->wbf, pts, n, m->norm_lo[0], m->norm_lo[1], m->norm_lo[2],
-        m->norm_inv[0], m->norm_inv[1], m->norm_inv[2], out);
-  else
-    MlpPredictKernel<3><<<std::max(blocks, 1), 256, 0, ctx->stream>>>(
-        m->wbf, pts, n, m->norm_lo[0], m->norm_lo[1], m->norm_lo[2],
-        m->norm_inv[0], m->norm_inv[1], m->norm_inv[2], out);
+  const uint64_t n_tiles = (n + 15) / 16;
+  const uint64_t blocks = std::min<uint64_t>((n_tiles + 3) / 4, 256u * 32u);
+  MlpPredictKernel<ND, kKeys><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
+      m->wbf, pts, n, m->norm_lo[0], m->norm_lo[1], m->norm_lo[2],
+      m->norm_inv[0], m->norm_inv[1], m->norm_inv[2], o);
   HD_HIP_TRY(hipGetLastError());
   return HD_OK;
+}
+
+int PredictInto(hd_context* ctx, const hd_mlp* m, const float* pts, uint64_t n,
+                float* out) {
+  const PredictOut o = {out, nullptr, nullptr};
+  return m->n_dims == 2 ? LaunchPredict<2, false>(ctx, m, pts, n, o)
+                        : LaunchPredict<3, false>(ctx, m, pts, n, o);
 }
 
 int TrainModel(hd_context* ctx, int n_dims, const float* d_a, uint64_t n_a,
@@ -622,16 +705,21 @@ int LearnedOrder(hd_context* ctx, const DirectedArgs& args,
     return Fail(HD_EINVAL, "model has %d dims, input %d", model->n_dims,
                 args.n_dims);
   const uint32_t n = (uint32_t)args.n_a;
-  float *pred = nullptr, *pred_s = nullptr;
+  uint16_t *key = nullptr, *key_s = nullptr;
   uint32_t *iota = nullptr, *order = nullptr;
-  HD_TRY(ContextArray(ctx, "lo_pred", n, &pred));
-  HD_TRY(ContextArray(ctx, "lo_pred_s", n, &pred_s));
+  HD_TRY(ContextArray(ctx, "lo_key", n, &key));
+  HD_TRY(ContextArray(ctx, "lo_key_s", n, &key_s));
   HD_TRY(ContextArray(ctx, "lo_iota", n, &iota));
   HD_TRY(ContextArray(ctx, "lo_order", n, &order));
   const int st = StageBegin(ctx, "learned_order");
-  HD_TRY(PredictInto(ctx, model, args.a, n, pred));
-  HD_TRY(IotaU32(ctx, iota, n));
-  HD_TRY(SortPairsDescF32(ctx, pred, pred_s, iota, order, n));
+  // Predictions quantised to 16-bit keys: two radix passes instead of four,
+  // and ties (within 2^-11 of a standard deviation) keep index order.
+  const PredictOut o = {nullptr, key, iota};
+  if (model->n_dims == 2)
+    HD_TRY((LaunchPredict<2, true>(ctx, model, args.a, n, o)));
+  else
+    HD_TRY((LaunchPredict<3, true>(ctx, model, args.a, n, o)));
+  HD_TRY(SortPairsU16(ctx, key, key_s, iota, order, n));
   StageEnd(ctx, st);
   *order_out = order;
   return HD_OK;

@@ -35,6 +35,15 @@ int SortPairsDescF32(hd_context* ctx, const float* k_in, float* k_out,
   });
 }
 
+int SortPairsU16(hd_context* ctx, const uint16_t* k_in, uint16_t* k_out,
+                 const uint32_t* v_in, uint32_t* v_out, uint32_t n) {
+  if (n == 0) return HD_OK;
+  return WithTemp(ctx, "radix_sort_pairs<u16>", [&](void* t, size_t& b) {
+    return rocprim::radix_sort_pairs(t, b, k_in, k_out, v_in, v_out, n, 0, 16,
+                                     ctx->stream);
+  });
+}
+
 int SortPairsU32(hd_context* ctx, const uint32_t* k_in, uint32_t* k_out,
                  const uint32_t* v_in, uint32_t* v_out, uint32_t n,
                  int end_bit) {

@@ -34,6 +34,52 @@ def test_gradient_matches_fp64_oracle(ctx, n_dims):
     np.testing.assert_allclose(grad, ref_grad, rtol=2e-3, atol=2e-4 * scale)
 
 
+def oracle_forward(params: np.ndarray, n_dims: int, x: np.ndarray) -> np.ndarray:
+    """FP64 network output per row of x: with target 0 and a batch of one,
+    the oracle's gradient of b3 (last parameter) is 2 f(x)."""
+    out = np.empty(x.shape[0])
+    zero = np.zeros(1, np.float32)
+    for i in range(x.shape[0]):
+        _, g = oracle.mlp_loss_grad(params, n_dims, 64, x[i : i + 1], zero)
+        out[i] = 0.5 * g[-1]
+    return out
+
+
+@pytest.mark.parametrize("n_dims", [2, 3])
+def test_inference_within_bf16_activation_bound(ctx, n_dims):
+    """Matrix-core inference rounds the sine activations to BF16 (unit
+    roundoff 2^-9) before the 64x64 layer. ReLU is 1-Lipschitz, so the
+    output error is at most 2^-9 sum_j |w3_j| sum_k |W2_jk| (plus FP32
+    rounding of the sine argument and accumulations, covered by 2^-14)."""
+    import torch
+
+    a = hd.gen.uniform(30 + n_dims, 40_000, n_dims)
+    b = (hd.gen.uniform(40 + n_dims, 3_000, n_dims) - np.float32(0.5)) * np.float32(0.05) \
+        + np.float32(0.4)
+    da = torch.from_numpy(a).cuda()
+    db = torch.from_numpy(np.ascontiguousarray(b)).cuda()
+    model = hd.Model.train(ctx, da, db, steps=200, batch=2048, n_samples=20_000)
+    pts = a[:2048]
+    pred = torch.empty(pts.shape[0], dtype=torch.float32, device="cuda")
+    model.predict(ctx, torch.from_numpy(pts).cuda(), pred)
+    got = pred.cpu().numpy().astype(np.float64)
+    w = bf16_round(model.params())
+    lo, inv = model.normalisation()
+    x = ((pts - lo) * inv).astype(np.float32)
+    ref = oracle_forward(w, n_dims, x)
+    off = 64 * n_dims + 64
+    w2 = np.abs(w[off : off + 64 * 64].astype(np.float64)).reshape(64, 64)
+    w3 = np.abs(w[off + 64 * 64 + 64 : off + 64 * 64 + 128].astype(np.float64))
+    bound = (2.0**-9 + 2.0**-14) * float(w3 @ w2.sum(axis=1))
+    err = np.abs(got - ref)
+    print(f"[mlp] {n_dims}D inference: max err {err.max():.3g}, bound {bound:.3g}")
+    assert err.max() <= bound
+    # The bound is a worst case; typical errors are far smaller, and a
+    # transposed or misplaced operand block would not be.
+    assert np.median(err) <= bound / 8
+    model.close()
+
+
 def test_training_fits_a_smooth_target(ctx):
     import torch
 
