@@ -43,11 +43,11 @@ MFMA_F32_TFLOPS = 157.3
 EB_KERNELS = {
     "EbScanKernel": {"full": "EbScanKernel<2,8,4>",
                      "stages": ("eb_seed", "eb_scan_r1", "eb_scan_r2", "eb_scan_rest",
-                                "eb_scan_r3", "eb_scan_r4", "eb_top", "eb_tail",
-                                "eb_tail_exact")},
+                                "eb_scan_r3", "eb_scan_r4", "eb_top", "eb_top_exact",
+                                "eb_tail", "eb_tail_exact")},
     "EbMfmaKernel": {"full": "EbMfmaKernel<2>",
                      "stages": ("eb_mfma_r1", "eb_mfma_r2", "eb_mfma_rest", "eb_mfma_r3",
-                                "eb_mfma_r4", "eb_mfma_tail")},
+                                "eb_mfma_r4", "eb_mfma_top", "eb_mfma_tail")},
 }
 MFMA_FLOP_PER_PAIR = 8
 

@@ -1121,9 +1121,15 @@ int LaunchMfmaSplit(hd_context* ctx, const float* a, const uint32_t* list,
   const uint32_t n_waves = (uint32_t)blocks * (kBlock / kWaveSize);
   // Segments of about 1024 steps (at least 16 of them): long items leave
   // the end of the launch to a few waves, short ones multiply the work
-  // spent after a hit until the other segments see it.
-  uint32_t n_segs = std::max(16u, t.n_steps / 1024u);
-  n_segs = std::min(n_segs, std::max(1u, t.n_steps / 256u));
+  // spent after a hit until the other segments see it. A short list gets
+  // more segments (down to 128 steps), enough for two claims of 16 entries
+  // per wave: a wave steps through its segment one dependent load after
+  // another, so the launch takes about seg_steps load latencies.
+  const uint32_t want = (uint32_t)std::min<uint64_t>(
+      0xFFFFFFFFull, (2ull * n_waves * 16u + n_list - 1) / n_list);
+  uint32_t n_segs = std::max(std::max(16u, t.n_steps / 1024u),
+                             std::min(want, t.n_steps / 128u));
+  n_segs = std::min(n_segs, std::max(1u, t.n_steps / 128u));
   const uint32_t seg_steps = (t.n_steps + n_segs - 1) / n_segs;
   n_segs = (t.n_steps + seg_steps - 1) / seg_steps;
   MfmaArgs args = {};
@@ -1182,6 +1188,59 @@ int ReadCount(hd_context* ctx, const unsigned int* d_count, uint32_t* out,
     std::memcpy(&f, &h[3], 4);
     *cmax2 = f;
   }
+  return HD_OK;
+}
+
+// Stage names of one kind of completion: filtered, exact follow-up, and
+// exact-only.
+struct CompletionStages {
+  const char* filtered;
+  const char* exact;
+  const char* plain;
+};
+
+// Completes the listed entries over all of B, raising *key. With the
+// matrix-core filter (B long enough for segments of 1024 steps and its
+// error bound below half of cmax2): segmented filtered scans, then exact
+// segmented scans of the entries that no segment could drop. Otherwise
+// exact segmented scans (at least min_segs segments per entry). ev[0], ev[1]
+// count the filtered and the follow-up evaluations, ev[2] the exact-only
+// ones. cmax2 is a value *key has reached (a lower bound of the current).
+template <int ND>
+int CompleteEntries(hd_context* ctx, const float* a, const uint32_t* list,
+                    const float* list_min, uint32_t n, const EbTarget& t,
+                    float cmax2, unsigned long long* key,
+                    unsigned long long* ev_filt, unsigned long long* ev_exact,
+                    unsigned long long* ev_plain, uint32_t min_segs,
+                    const CompletionStages& names, bool* filtered,
+                    uint64_t* n_exact) {
+  *filtered = false;
+  if (n == 0) return HD_OK;
+  const bool filt =
+      t.bt != nullptr && t.n_steps >= 4096u && t.err < 0.5f * cmax2;
+  *filtered = filt;
+  if (!filt) {
+    const int st = StageBegin(ctx, names.plain);
+    HD_TRY((LaunchSplit<ND>(ctx, a, list, list_min, n, t.bx, t.by, t.bz,
+                           t.n_steps, key, ev_plain, min_segs)));
+    StageEnd(ctx, st);
+    return HD_OK;
+  }
+  uint32_t* cand = nullptr;
+  unsigned int* cand_count = nullptr;
+  HD_TRY(ContextArray(ctx, "eb_cand", n, &cand));
+  HD_TRY(ContextArray(ctx, "eb_cand_count", 1, &cand_count));
+  const int st = StageBegin(ctx, names.filtered);
+  HD_TRY((LaunchMfmaSplit<ND>(ctx, a, list, list_min, n, t, key, ev_filt, cand,
+                              cand_count)));
+  StageEnd(ctx, st);
+  uint32_t n_cand = 0;
+  HD_TRY(ReadCount(ctx, cand_count, &n_cand));
+  *n_exact += n_cand;
+  const int st_x = StageBegin(ctx, names.exact);
+  HD_TRY((LaunchSplit<ND>(ctx, a, cand, nullptr, n_cand, t.bx, t.by, t.bz,
+                         t.n_steps, key, ev_exact, 16u)));
+  StageEnd(ctx, st_x);
   return HD_OK;
 }
 
@@ -1296,7 +1355,7 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
       extra_begin = extra_end;
     }
     uint32_t n_esc = 0;
-    HD_TRY(ReadCount(ctx, counters + 1, &n_esc));
+    HD_TRY(ReadCount(ctx, counters + 1, &n_esc, key, &cmax2));
     if (n_esc == 0) {
       begin = end = 0;
       continue;
@@ -1305,12 +1364,17 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
     HD_TRY(SortPairsDescF32(ctx, min_out, min_sorted, idx_out, idx_sorted,
                             n_esc));
     StageEnd(ctx, st_sort);
+    // Exact completions of the escalations with the largest partial minima
+    // (the likeliest to raise cmax2).
     const uint32_t k_top = std::min<uint32_t>(n_esc, 256u);
     rs->top_points += k_top;
-    const int st_top = StageBegin(ctx, "eb_top");
-    HD_TRY((LaunchSplit<ND>(ctx, a, idx_sorted, min_sorted, k_top, t.bx, t.by,
-                           t.bz, t.n_steps, key, ev + kMaxRounds)));
-    StageEnd(ctx, st_top);
+    bool top_filt = false;
+    HD_TRY((CompleteEntries<ND>(
+        ctx, a, idx_sorted, min_sorted, k_top, t, cmax2, key,
+        ev + kMaxRounds + 4, ev + kMaxRounds + 5, ev + kMaxRounds, 1u,
+        {"eb_mfma_top", "eb_top_exact", "eb_top"}, &top_filt,
+        &rs->top_exact_points)));
+    if (top_filt) rs->top_filtered_points += k_top;
     // The sorted escalations become the next round's list; the next round
     // writes its sort into the buffer this round read from.
     mode = kOrderArray;
@@ -1332,36 +1396,13 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
     rs->tail_points += end - begin;
     const uint32_t n_tail = end - begin;
     const float* tail_min = list_min ? list_min + begin : nullptr;
-    bool filt = false;
-    if (t.bt != nullptr && t.n_steps >= 4096u) {
+    if (t.bt != nullptr && t.n_steps >= 4096u)
       HD_TRY(ReadCount(ctx, nullptr, nullptr, key, &cmax2));
-      filt = t.err < 0.5f * cmax2;
-    }
-    rs->tail_filtered = filt;
-    if (filt) {
-      uint32_t* cand = nullptr;
-      unsigned int* cand_count = nullptr;
-      HD_TRY(ContextArray(ctx, "eb_cand", n_tail, &cand));
-      HD_TRY(ContextArray(ctx, "eb_cand_count", 1, &cand_count));
-      const int st_tail = StageBegin(ctx, "eb_mfma_tail");
-      HD_TRY((LaunchMfmaSplit<ND>(ctx, a, list + begin, tail_min, n_tail, t,
-                                  key, ev + kMaxRounds + 1, cand,
-                                  cand_count)));
-      StageEnd(ctx, st_tail);
-      uint32_t n_cand = 0;
-      HD_TRY(ReadCount(ctx, cand_count, &n_cand));
-      rs->exact_points += n_cand;
-      const int st_exact = StageBegin(ctx, "eb_tail_exact");
-      HD_TRY((LaunchSplit<ND>(ctx, a, cand, nullptr, n_cand, t.bx, t.by, t.bz,
-                             t.n_steps, key, ev + kMaxRounds + 2, 16u)));
-      StageEnd(ctx, st_exact);
-    } else {
-      const int st_tail = StageBegin(ctx, "eb_tail");
-      HD_TRY((LaunchSplit<ND>(ctx, a, list + begin, tail_min, n_tail, t.bx,
-                             t.by, t.bz, t.n_steps, key, ev + kMaxRounds + 1,
-                             16u)));
-      StageEnd(ctx, st_tail);
-    }
+    HD_TRY((CompleteEntries<ND>(
+        ctx, a, list + begin, tail_min, n_tail, t, cmax2, key,
+        ev + kMaxRounds + 1, ev + kMaxRounds + 2, ev + kMaxRounds + 1, 16u,
+        {"eb_mfma_tail", "eb_tail_exact", "eb_tail"}, &rs->tail_filtered,
+        &rs->exact_points)));
   }
   return HD_OK;
 }
@@ -1437,7 +1478,13 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
       const bool filt = (rs.mfma_rounds >> r) & 1u;
       StageCount(ctx, RoundStage(r, filt), h[1 + r], rs.round_points[r]);
     }
-    StageCount(ctx, "eb_top", h[1 + kMaxRounds], rs.top_points);
+    if (rs.top_points > rs.top_filtered_points)
+      StageCount(ctx, "eb_top", h[1 + kMaxRounds],
+                 rs.top_points - rs.top_filtered_points);
+    if (rs.top_filtered_points != 0) {
+      StageCount(ctx, "eb_mfma_top", h[5 + kMaxRounds], rs.top_filtered_points);
+      StageCount(ctx, "eb_top_exact", h[6 + kMaxRounds], rs.top_exact_points);
+    }
     if (rs.rest_points != 0)
       StageCount(ctx, rs.rest_filtered ? "eb_mfma_rest" : "eb_scan_rest",
                  h[4 + kMaxRounds], rs.rest_points);

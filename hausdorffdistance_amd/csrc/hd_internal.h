@@ -200,11 +200,17 @@ struct EbTarget {
 // the tail, the tail's exact follow-up and the part of the order a sampled
 // first round left for the second.
 constexpr int kMaxRounds = 4;
-constexpr int kEbCounters = kMaxRounds + 4;
+// Evaluation counters of the early-break pipeline: one per budgeted round,
+// then exact top completions, tail (filtered or exact-only), tail exact
+// follow-up, the rest of a sampled first round, filtered top completions
+// and their exact follow-up.
+constexpr int kEbCounters = kMaxRounds + 6;
 struct EbRoundStats {
   uint64_t round_points[kMaxRounds] = {0, 0, 0, 0};
   uint32_t mfma_rounds = 0;
   uint64_t top_points = 0;
+  uint64_t top_filtered_points = 0;  // top completions on the filter
+  uint64_t top_exact_points = 0;     // ... that no segment could drop
   uint64_t tail_points = 0;
   uint64_t exact_points = 0;    // tail entries no segment could drop
   bool tail_filtered = false;
