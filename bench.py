@@ -45,9 +45,12 @@ EB_KERNELS = {
                      "stages": ("eb_seed", "eb_scan_r1", "eb_scan_r2", "eb_scan_rest",
                                 "eb_scan_r3", "eb_scan_r4", "eb_top", "eb_top_exact",
                                 "eb_tail", "eb_tail_exact")},
-    "EbMfmaKernel": {"full": "EbMfmaKernel<2>",
-                     "stages": ("eb_mfma_r1", "eb_mfma_r2", "eb_mfma_rest", "eb_mfma_r3",
-                                "eb_mfma_r4", "eb_mfma_top", "eb_mfma_tail")},
+    "EbMfmaKernel": {"full": "EbMfmaKernel<2, false>",
+                     "stages": ("eb_mfma_r1", "eb_mfma_r2", "eb_mfma_rest", "eb_mfma_top",
+                                "eb_mfma_tail")},
+    # The 1024-step round, split over four 256-step segments per point.
+    "EbMfmaKernelGrouped": {"full": "EbMfmaKernel<2, true>",
+                            "stages": ("eb_mfma_r3", "eb_mfma_r4")},
 }
 MFMA_FLOP_PER_PAIR = 8
 
@@ -110,7 +113,7 @@ def eb_roofline(stages: dict, steps: int) -> dict:
             traffic = tr.get("hbm_bytes_per_launch")
     other = {n: {"ms_per_step": v[0] / max(1, steps), "launches_per_step": v[1] / max(1, steps),
                  "evals_per_step": v[2] / max(1, steps)} for n, v in per.items()}
-    if name == "EbMfmaKernel":
+    if name.startswith("EbMfmaKernel"):
         flop = MFMA_FLOP_PER_PAIR * evals / launches
         achieved = flop / t_kernel / 1e12 if t_kernel > 0 else None
         return {

@@ -470,6 +470,16 @@ __global__ void SeedListKernel(const uint32_t* order, Feistel perm,
 // point's running minimum lives in 4 lanes of one register per tile. Slot
 // metadata (A index, order position, first step) lives in lane s for slot
 // s = 16 t + j.
+//
+// Grouped budgeted rounds (n_groups = G > 0): the budget is split over
+// W = n_segs / G segments of seg_steps steps; position p belongs to group
+// p mod G and is scanned by the W segments s with s mod G = p mod G (disjoint
+// B ranges, W * seg_steps = budget steps in all), concurrently. A hit in any
+// of them drops the entry (broken[p]); a segment that completes without one
+// folds its bound (approx + E, rounded up) into pmin[p] and counts gdone[p];
+// the W-th count escalates the entry with pmin[p]. The same work as one
+// budgeted scan, in W times as many shorter items: the launch fills the
+// chip and its drain is a quarter as long.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 struct MfmaArgs {
@@ -479,12 +489,18 @@ struct MfmaArgs {
   Feistel perm_a;
   int order_mode;
   uint32_t pos_end;
-  unsigned int* cursors;  // per segment: next order position
+  unsigned int* cursors;  // per segment: next order position (grouped:
+                          // next index within the segment's group)
   uint32_t claim;         // positions per cursor claim (<= 64)
   uint32_t seg_steps;     // steps per segment
   uint32_t n_segs;        // 1: budgeted round over all of B
+  uint32_t n_groups;      // 0: every segment serves every position; else a
+                          // budgeted round split over n_segs / n_groups
+                          // segments per position (see EbMfmaKernel)
+  uint32_t pos_begin;     // grouped: first position
   uint32_t* broken;       // n_segs > 1, per position: 1 once a hit is known
   uint32_t* gdone;        // n_segs > 1, per position: segments completed
+  uint32_t* pmin;         // grouped, per position: min of segment bounds
   uint32_t* cand;         // n_segs > 1: A indices that completed every
   unsigned int* cand_count;  // segment without a hit (exact scan follows)
   const float* bt;        // B tiles, 64 floats each, 4 per step
@@ -529,7 +545,7 @@ HD_DEV float FilterUpper(uint32_t cm_bits, float err) {
   return __double2float_ru(t);
 }
 
-template <int ND>
+template <int ND, bool kGrouped>
 __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
   const uint32_t lane = LaneId();
   const uint32_t lane4 = lane << 2;
@@ -539,6 +555,8 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
   const uint32_t n_waves = gridDim.x * (kBlock / kWaveSize);
   const uint32_t n_steps = args.n_steps;
   const bool split = args.n_segs > 1;
+  constexpr bool grouped = kGrouped;
+  const uint32_t per_entry = grouped ? args.n_segs / args.n_groups : args.n_segs;
 
   // bop: operand lanes; mn: running minima (FP32 bits as signed integers);
   // cna: 3D, C - |a'|^2 of the lane's point.
@@ -592,6 +610,16 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
     // complete after seg_len steps.
     const uint32_t limit = split ? seg_len : args.budget_steps;
     unsigned int* cursor = args.cursors + seg;
+    // The cursor counts q in [.., q_end); position = q_off + q_stride q
+    // (grouped: the group's positions g, g + G, ...; else q itself).
+    uint32_t q_off = 0, q_stride = 1, q_end = args.pos_end;
+    if (grouped) {
+      q_off = args.pos_begin + seg % args.n_groups;
+      q_stride = args.n_groups;
+      q_end = args.pos_end > q_off
+                  ? (args.pos_end - q_off + q_stride - 1) / q_stride
+                  : 0u;
+    }
     uint32_t step =
         seg_begin + (uint32_t)(((uint64_t)(kk / args.n_segs) * 7919u) % seg_len);
     live = 0;
@@ -625,12 +653,13 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
           uint32_t base = 0;
           if (lane == 0) base = atomicAdd(cursor, args.claim);
           base = Uniform(base);
-          if (base >= args.pos_end) {
+          if (base >= q_end) {
             drained = true;
             break;
           }
-          const uint32_t pos = base + lane;
-          bool valid = lane < args.claim && pos < args.pos_end;
+          const uint32_t q = base + lane;
+          bool valid = lane < args.claim && q < q_end;
+          const uint32_t pos = valid ? q_off + q_stride * q : 0u;
           if (valid && args.list_min != nullptr)
             valid = !(__float_as_uint(args.list_min[pos]) < cm);
           if (valid && split)
@@ -802,7 +831,7 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
           poll ? __ballot(flag != 0u) & live & ~hit : 0ull;
       unsigned long long due =
           __ballot(gstep - m_start >= limit) & live & ~(hit | gone);
-      if (split && due != 0ull) {
+      if (split && !kGrouped && due != 0ull) {
         // Segment scanned without a hit: count it; an entry that completes
         // every segment goes to the exact scan.
         if ((due >> lane) & 1ull) {
@@ -821,21 +850,44 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
           if (ND == 3) v -= (double)cna[t];
           if (kc == (uint32_t)t) mine = v;
         }
+        const double ub = (mine > 0.0 ? mine : 0.0) + (double)args.err;
+        float esc_val = __double2float_ru(ub);
+        unsigned long long esc_m = due;
+        if constexpr (kGrouped) {
+          // Fold this segment's bound into the entry's, then count the
+          // segment. The count's increment depends on the value the atomic
+          // min returned, so it is issued after the min has been performed
+          // at the device coherence point, and the count that completes the
+          // entry reads the minimum over all its segments there.
+          bool last = false;
+          if ((due >> lane) & 1ull) {
+            const uint32_t was = atomicMin(&args.pmin[m_pos], __float_as_uint(esc_val));
+            const uint32_t prev = __hip_atomic_fetch_add(
+                &args.gdone[m_pos], 1u + (was >> 31), __ATOMIC_RELAXED,
+                __HIP_MEMORY_SCOPE_AGENT);
+            if (prev + 1u == per_entry) {
+              last = true;
+              esc_val = __uint_as_float(__hip_atomic_load(
+                  &args.pmin[m_pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            }
+          }
+          esc_m = __ballot(last);
+        }
         // Stage the escalations in lanes [n_stg, n_stg + n_esc) (a forward
-        // permutation: due slots in order, the other lanes after them) and
-        // append 64 at a time, so the shared counter sees one atomic per 64.
-        const uint32_t n_esc = (uint32_t)__popcll(due);
+        // permutation: escalating slots in order, the other lanes after
+        // them) and append 64 at a time, so the shared counter sees one
+        // atomic per 64.
+        const uint32_t n_esc = (uint32_t)__popcll(esc_m);
         if (n_stg + n_esc > (uint32_t)kWaveSize) flush();
-        const bool esc = (due >> lane) & 1ull;
+        const bool esc = (esc_m >> lane) & 1ull;
         const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(due >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)due, 0u));
+            (uint32_t)(esc_m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)esc_m, 0u));
         const uint32_t dest =
             (esc ? n_stg + below : n_stg + n_esc + (lane - below)) & 63u;
-        const double ub = (mine > 0.0 ? mine : 0.0) + (double)args.err;
         const int d4 = (int)(dest << 2);
         const uint32_t p_idx = (uint32_t)__builtin_amdgcn_ds_permute(d4, (int)m_idx);
         const float p_min = __int_as_float(__builtin_amdgcn_ds_permute(
-            d4, __float_as_int(__double2float_ru(ub))));
+            d4, __float_as_int(esc_val)));
         if (lane >= n_stg && lane < n_stg + n_esc) {
           stg_idx = p_idx;
           stg_min = p_min;
@@ -1036,13 +1088,13 @@ int PrepareFilter(hd_context* ctx, const float* mbr_a, const float* mbr_b,
   return HD_OK;
 }
 
-template <int ND>
+template <int ND, bool kGrouped = false>
 int MfmaResidentBlocks(hd_context* ctx) {
   static int per_cu[4] = {0, 0, 0, 0};
   if (per_cu[ND] == 0) {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, EbMfmaKernel<ND>, kBlock, 0) != hipSuccess || n < 1)
+            &n, EbMfmaKernel<ND, kGrouped>, kBlock, 0) != hipSuccess || n < 1)
       n = 1;
     per_cu[ND] = n;
   }
@@ -1052,6 +1104,9 @@ int MfmaResidentBlocks(hd_context* ctx) {
 // Filtered budgeted round over order positions [pos_begin, pos_end); every
 // point that is not proved below cmax2 within budget_steps is escalated
 // with an upper bound of its partial minimum.
+// Segment length of grouped budgeted rounds.
+constexpr uint32_t kGroupSteps = 256u;
+
 template <int ND>
 int LaunchMfma(hd_context* ctx, const float* a, int order_mode,
                const uint32_t* order, const float* list_min,
@@ -1090,16 +1145,55 @@ int LaunchMfma(hd_context* ctx, const float* a, int order_mode,
   args.esc_count = counters + 1;
   args.evals = evals;
   const uint32_t n_entries = pos_end - pos_begin;
-  int blocks = MfmaResidentBlocks<ND>(ctx);
-  // At least 8 entries per slot: with fewer, most of the round is the
-  // drain, where tiles with a few live slots still cost all their MFMAs.
-  const uint32_t waves_needed = (n_entries + 511u) / 512u;
+  // Too few entries for 8 per slot of a full launch: a budget of W >= 2
+  // segments of kGroupSteps steps runs grouped (see EbMfmaKernel), W items
+  // per entry. (With enough entries, splitting would only add the steps
+  // that segments spend before they see another's hit.)
+  const uint32_t w = budget_steps / kGroupSteps;
+  const uint64_t full_items =
+      (uint64_t)MfmaResidentBlocks<ND, false>(ctx) * (kBlock / kWaveSize) * 512u;
+  const bool grouped = w >= 2 && budget_steps % kGroupSteps == 0 &&
+                       t.n_steps / budget_steps >= 1 && n_entries < full_items;
+  uint32_t n_groups = 1, items = n_entries;
+  if (grouped) {
+    n_groups = t.n_steps / budget_steps;
+    args.n_groups = n_groups;
+    args.pos_begin = pos_begin;
+    args.seg_steps = kGroupSteps;
+    args.n_segs = n_groups * w;
+    items = n_entries * w;
+    HD_TRY(ContextArray(ctx, "eb_mfma_cursors", args.n_segs, &args.cursors));
+    HD_TRY(FillU32(ctx, args.cursors, 0u, args.n_segs));
+    HD_TRY(ContextArray(ctx, "eb_broken", pos_end, &args.broken));
+    HD_TRY(ContextArray(ctx, "eb_gdone", pos_end, &args.gdone));
+    HD_TRY(ContextArray(ctx, "eb_pmin", pos_end, &args.pmin));
+    HD_TRY(FillU32(ctx, args.broken + pos_begin, 0u, n_entries));
+    HD_TRY(FillU32(ctx, args.gdone + pos_begin, 0u, n_entries));
+    HD_TRY(FillU32(ctx, args.pmin + pos_begin, kFltMaxBits, n_entries));
+  }
+  int blocks = grouped ? MfmaResidentBlocks<ND, true>(ctx)
+                       : MfmaResidentBlocks<ND, false>(ctx);
+  // At least 8 items per slot: with fewer, most of the round is the drain,
+  // where tiles with a few live slots still cost all their MFMAs.
+  const uint32_t waves_needed = (uint32_t)std::min<uint64_t>(
+      0xFFFFFFFFull, ((uint64_t)items + 511u) / 512u);
   if ((uint32_t)blocks * 4u > waves_needed)
     blocks = (int)std::max(1u, (waves_needed + 3u) / 4u);
   const uint32_t n_waves = (uint32_t)blocks * (kBlock / kWaveSize);
-  args.claim = std::min<uint32_t>(
-      kWaveSize, std::max<uint32_t>(16u, (n_entries + n_waves - 1) / n_waves));
-  EbMfmaKernel<ND><<<blocks, kBlock, 0, ctx->stream>>>(args);
+  if (grouped) {
+    // Per segment: the group's positions over the waves that share it.
+    const uint32_t per_seg = std::max(1u, n_waves / args.n_segs);
+    const uint32_t per_group = (n_entries + n_groups - 1) / n_groups;
+    args.claim = std::min<uint32_t>(
+        kWaveSize, std::max<uint32_t>(16u, (per_group + per_seg - 1) / per_seg));
+  } else {
+    args.claim = std::min<uint32_t>(
+        kWaveSize, std::max<uint32_t>(16u, (n_entries + n_waves - 1) / n_waves));
+  }
+  if (grouped)
+    EbMfmaKernel<ND, true><<<blocks, kBlock, 0, ctx->stream>>>(args);
+  else
+    EbMfmaKernel<ND, false><<<blocks, kBlock, 0, ctx->stream>>>(args);
   HD_HIP_TRY(hipGetLastError());
   return HD_OK;
 }
@@ -1165,7 +1259,7 @@ int LaunchMfmaSplit(hd_context* ctx, const float* a, const uint32_t* list,
   const uint32_t per_seg = std::max(1u, n_waves / n_segs);
   args.claim = std::min<uint32_t>(
       kWaveSize, std::max<uint32_t>(16u, (n_list + per_seg - 1) / per_seg));
-  EbMfmaKernel<ND><<<blocks, kBlock, 0, ctx->stream>>>(args);
+  EbMfmaKernel<ND, false><<<blocks, kBlock, 0, ctx->stream>>>(args);
   HD_HIP_TRY(hipGetLastError());
   return HD_OK;
 }

@@ -152,6 +152,24 @@ def test_matrix_core_filter_with_all_points_tied(ctx):
     assert o.argmax == 0 and r.ab.cmax2_bits == np.float32(2.0 ** -18).view(np.uint32)
 
 
+def test_matrix_core_filter_grouped_round_with_all_points_tied(ctx):
+    # A 260 x 260 lattice (step 2^-7) against itself shifted by half a step:
+    # every A point's nearest B point is at exactly 2^-8, so no point breaks
+    # and every one escalates through the rounds, including the 1024-step
+    # round, which runs split over four 256-step segments per point (the
+    # completion count and the minimum over the segments decide its
+    # escalation). Exact answer: cmax2 = 2^-16, arg-max 0, its neighbour 0.
+    g = np.arange(260, dtype=np.float32) * np.float32(2.0 ** -7)
+    xx, yy = np.meshgrid(g, g, indexing="ij")
+    a = np.stack([xx.ravel(), yy.ravel()], 1).astype(np.float32)
+    b = a + np.array([2.0 ** -8, 0.0], np.float32)
+    print("[filter] tied lattice, grouped round", a.shape, flush=True)
+    r, st = _eb_with_stages(ctx, a, b, symmetric=False)
+    assert st.get("eb_mfma_r3", {}).get("points", 0) > 0, sorted(st)
+    assert triple(r.ab) == {"cmax2_bits": int(np.float32(2.0 ** -16).view(np.uint32)),
+                            "argmax": 0, "nn": 0}
+
+
 def test_matrix_core_filter_far_from_origin(ctx):
     # A large common offset: the filter centres coordinates before the
     # matrix products; results must not change.
