@@ -123,4 +123,11 @@ def test_training_lowers_loss_and_order_keeps_bits(ctx):
     # Without an explicit model the library trains one for these inputs.
     r = hd.directed_device(ctx, hd.Variant.EB, da, db, learned_order=True)
     assert (r.cmax2_bits, r.argmax) == (o.cmax2_bits, o.argmax)
+    # A model trained on other inputs: its farthest labelled points index
+    # another A (those past the end are skipped); the bits do not change.
+    a2 = np.ascontiguousarray(a[:30_000][::-1])
+    o2 = oracle.eb_directed(a2, b)
+    r2 = hd.directed_device(ctx, hd.Variant.EB, torch.from_numpy(a2).cuda(), db,
+                            learned_order=True, model=model)
+    assert (r2.cmax2_bits, r2.argmax, r2.nn) == (o2.cmax2_bits, o2.argmax, o2.nn)
     model.close()
