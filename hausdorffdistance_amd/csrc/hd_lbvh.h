@@ -40,6 +40,7 @@ struct LbvhIndex {
   LbvhDev dev;
   LbvhFrame frame;
   int n_dims = 2;
+  const uint32_t* codes = nullptr;  // sorted 32-bit Morton codes of B
 };
 
 struct LbvhQueryArgs {
@@ -58,15 +59,23 @@ struct LbvhQueryArgs {
   uint32_t* esc_idx;       // traversal-stack overflows
   float* esc_min;
   unsigned int* esc_count;
+  const uint32_t* start_leaf;  // optional, by position: the leaf where the
+                               // query's Morton code falls among B's codes
+  uint32_t stride;             // > 1: query k takes position k * stride
 };
 
 int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
               uint32_t leaf_size, LbvhIndex* out);
 int LaunchLbvhQuery(hd_context* ctx, const LbvhQueryArgs& args);
-// Permutation of pts sorted by Morton code in frame f (device array owned
-// by the context).
+// Permutation of pts sorted by the top 32 bits of their Morton codes in
+// frame f (device array owned by the context).
 int MortonOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
                 const LbvhFrame& f, uint32_t** order_out);
+// start[p] = leaf of tree holding the first B code >= the Morton code of
+// query q[order[p]] (clamped to the last leaf); order should be a Morton
+// order of the queries, for coherent searches.
+int StartLeaves(hd_context* ctx, const LbvhIndex& tree, const float* q,
+                const uint32_t* order, uint32_t n, uint32_t* start);
 int LbvhNearestNeighbors(hd_context* ctx, int n_dims, const float* q,
                          uint64_t n_q, const float* ref, uint64_t n_ref,
                          float* out_d2, uint32_t* out_idx);

@@ -6,8 +6,8 @@
 // cell's points with early break (src/rt/shaders/
 // shaders_nn_uniform_grid_2d.cu:16-106). CDNA4 has no ray-tracing units, so
 // here the hierarchy is built and traversed on the shader cores:
-//  * build: 64-bit Morton codes of B (21 bits per axis in 3D, 32 in 2D),
-//    one radix sort, leaves = runs of kLeaf consecutive sorted points,
+//  * build: 32-bit Morton codes of B (16 bits per axis in 2D, 10-11 in 3D;
+//    equal codes ordered by index), one radix sort, leaves = runs of kLeaf consecutive sorted points,
 //    internal nodes by Karras (HPG 2012) with index tie-breaking for equal
 //    codes, bottom-up boxes with one arrival counter per node;
 //  * node record: both child boxes inline plus the two child ids, so a visit
@@ -55,25 +55,36 @@ HD_DEV unsigned long long Spread3x21(uint32_t v) {
   return x;
 }
 
+// Morton code of point p in frame f (64 bits in 2D, 63 in 3D).
 template <int ND>
-__global__ void MortonKernel(const float* __restrict__ b, uint32_t n,
-                             LbvhFrame f, unsigned long long* __restrict__ keys,
-                             uint32_t* __restrict__ vals) {
+HD_DEV unsigned long long MortonCode(const float* p, const LbvhFrame& f) {
+  uint32_t q[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const float t = (p[d] - f.lo[d]) * f.scale[d];
+    q[d] = (uint32_t)fminf(fmaxf(t, 0.0f), f.qmax);
+  }
+  if constexpr (ND == 2)
+    return Spread2x32(q[0]) | (Spread2x32(q[1]) << 1);
+  else
+    return Spread3x21(q[0]) | (Spread3x21(q[1]) << 1) | (Spread3x21(q[2]) << 2);
+}
+
+// The top 32 bits of the Morton code: 16 bits per axis in 2D, 10-11 in 3D
+// (a cell of 2^-16 or 2^-10 of B's extent; points sharing one are ordered
+// by index). Four radix passes instead of eight.
+template <int ND>
+HD_DEV uint32_t MortonCode32(const float* p, const LbvhFrame& f) {
+  return (uint32_t)(MortonCode<ND>(p, f) >> (ND == 2 ? 32 : 31));
+}
+
+template <int ND>
+__global__ void MortonKernel32(const float* __restrict__ b, uint32_t n,
+                               LbvhFrame f, uint32_t* __restrict__ keys,
+                               uint32_t* __restrict__ vals) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
-    uint32_t q[ND];
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      const float t = (b[(uint64_t)i * ND + d] - f.lo[d]) * f.scale[d];
-      q[d] = (uint32_t)fminf(fmaxf(t, 0.0f), f.qmax);
-    }
-    unsigned long long code;
-    if constexpr (ND == 2)
-      code = Spread2x32(q[0]) | (Spread2x32(q[1]) << 1);
-    else
-      code = Spread3x21(q[0]) | (Spread3x21(q[1]) << 1) |
-             (Spread3x21(q[2]) << 2);
-    keys[i] = code;
+    keys[i] = MortonCode32<ND>(b + (uint64_t)i * ND, f);
     vals[i] = i;
   }
 }
@@ -97,16 +108,16 @@ __global__ void GatherSortedKernel(const float* __restrict__ b, uint32_t n,
 
 // Karras' prefix length between leaves i and j (codes of their first
 // points), with the leaf index as tie-breaker for equal codes.
-HD_DEV int Delta(const unsigned long long* codes, int n_leaf, uint32_t leaf,
-                 int i, int j) {
+HD_DEV int Delta(const uint32_t* codes, int n_leaf, uint32_t leaf, int i,
+                 int j) {
   if (j < 0 || j >= n_leaf) return -1;
-  const unsigned long long ci = codes[(uint64_t)i * leaf];
-  const unsigned long long cj = codes[(uint64_t)j * leaf];
-  if (ci == cj) return 64 + __clz((uint32_t)(i ^ j));
-  return __clzll(ci ^ cj);
+  const uint32_t ci = codes[(uint64_t)i * leaf];
+  const uint32_t cj = codes[(uint64_t)j * leaf];
+  if (ci == cj) return 32 + __clz((uint32_t)(i ^ j));
+  return __clz(ci ^ cj);
 }
 
-__global__ void KarrasKernel(const unsigned long long* __restrict__ codes,
+__global__ void KarrasKernel(const uint32_t* __restrict__ codes,
                              int n_leaf, uint32_t leaf,
                              uint32_t* __restrict__ child,
                              uint32_t* __restrict__ node_parent,
@@ -145,20 +156,13 @@ __global__ void KarrasKernel(const unsigned long long* __restrict__ codes,
   }
 }
 
-// Leaf boxes, then up the tree: the second child to arrive at a node writes
-// the node record (both child boxes inline) and continues upwards.
+// Leaf boxes (lo[ND], hi[ND] at stride 6), before the refit pass.
 template <int ND>
-__global__ void RefitKernel(int n_leaf, uint32_t leaf, uint32_t n,
-                            const float* __restrict__ x,
-                            const float* __restrict__ y,
-                            const float* __restrict__ z,
-                            const uint32_t* __restrict__ child,
-                            const uint32_t* __restrict__ node_parent,
-                            const uint32_t* __restrict__ leaf_parent,
-                            uint32_t* __restrict__ arrive,
-                            float* __restrict__ leaf_box,
-                            float* __restrict__ node_box,
-                            LbvhNode* __restrict__ nodes) {
+__global__ void LeafBoxKernel(int n_leaf, uint32_t leaf, uint32_t n,
+                              const float* __restrict__ x,
+                              const float* __restrict__ y,
+                              const float* __restrict__ z,
+                              float* __restrict__ leaf_box) {
   for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n_leaf;
        li += gridDim.x * blockDim.x) {
     float lo[3] = {kFltMax, kFltMax, kFltMax};
@@ -178,45 +182,113 @@ __global__ void RefitKernel(int n_leaf, uint32_t leaf, uint32_t n,
       leaf_box[(uint64_t)li * 6 + d] = lo[d];
       leaf_box[(uint64_t)li * 6 + 3 + d] = hi[d];
     }
+  }
+}
+
+// Up the tree from every leaf: the second child subtree to arrive at a node
+// writes the node record (both child boxes inline) and continues upwards
+// with the union. A subtree's box reaches its sibling's thread through the
+// node's mailbox (pub, 2 ND words per child slot): written with atomic
+// exchanges whose results the arrival count's increment depends on, so the
+// count is raised only after the box is at the device coherence point, and
+// read with atomic loads by the thread that saw the other arrival. No
+// acquire or release fence: at device scope those write back and
+// invalidate the L2.
+template <int ND>
+__global__ void RefitKernel(int n_leaf, const float* __restrict__ leaf_box,
+                            const uint32_t* __restrict__ child,
+                            const uint32_t* __restrict__ node_parent,
+                            const uint32_t* __restrict__ leaf_parent,
+                            uint32_t* __restrict__ arrive,
+                            uint32_t* __restrict__ pub,
+                            LbvhNode* __restrict__ nodes) {
+  for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n_leaf;
+       li += gridDim.x * blockDim.x) {
     if (n_leaf == 1) continue;
+    float lo[ND], hi[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      lo[d] = leaf_box[(uint64_t)li * 6 + d];
+      hi[d] = leaf_box[(uint64_t)li * 6 + 3 + d];
+    }
+    uint32_t me = (uint32_t)li | kLeafFlag;
     uint32_t node = leaf_parent[li];
     while (true) {
-      // Release this subtree's box, acquire the sibling's.
-      const uint32_t prev = __hip_atomic_fetch_add(
-          &arrive[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-      if (prev == 0) break;  // the sibling is not done yet
-      LbvhNode rec;
-      float ulo[3] = {kFltMax, kFltMax, kFltMax};
-      float uhi[3] = {-kFltMax, -kFltMax, -kFltMax};
+      const uint32_t c0 = child[2 * node], c1 = child[2 * node + 1];
+      const int slot = c0 == me ? 0 : 1;
+      uint32_t* mail = pub + ((uint64_t)node * 2 + slot) * (2 * ND);
+      uint32_t dep = 0xFFFFFFFFu;
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const uint32_t ch = child[2 * node + c];
-        const float* box = (ch & kLeafFlag)
-                               ? &leaf_box[(uint64_t)(ch & ~kLeafFlag) * 6]
-                               : &node_box[(uint64_t)ch * 6];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const float l = d < ND ? box[d] : 0.f;
-          const float h = d < ND ? box[3 + d] : 0.f;
-          rec.lo[c][d] = l;
-          rec.hi[c][d] = h;
-          if (d < ND) {
-            ulo[d] = fminf(ulo[d], l);
-            uhi[d] = fmaxf(uhi[d], h);
-          }
-        }
-        rec.child[c] = ch;
+      for (int d = 0; d < ND; ++d) {
+        dep &= __hip_atomic_exchange(&mail[d], __float_as_uint(lo[d]),
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        dep &= __hip_atomic_exchange(&mail[ND + d], __float_as_uint(hi[d]),
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+      // dep is all ones only if every old word was a NaN pattern; either
+      // increment works (only prev == 0 is tested), the dependency is what
+      // orders the count after the exchanges.
+      const uint32_t prev = __hip_atomic_fetch_add(
+          &arrive[node], 1u + (dep == 0xFFFFFFFFu ? 1u : 0u), __ATOMIC_RELAXED,
+          __HIP_MEMORY_SCOPE_AGENT);
+      if (prev == 0) break;  // the sibling is not done yet
+      const uint32_t* other = pub + ((uint64_t)node * 2 + (1 - slot)) * (2 * ND);
+      float olo[ND], ohi[ND];
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        olo[d] = __uint_as_float(__hip_atomic_load(&other[d], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT));
+        ohi[d] = __uint_as_float(__hip_atomic_load(
+            &other[ND + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      }
+      LbvhNode rec;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float l0 = d < ND ? (slot == 0 ? lo[d] : olo[d]) : 0.f;
+        const float h0 = d < ND ? (slot == 0 ? hi[d] : ohi[d]) : 0.f;
+        const float l1 = d < ND ? (slot == 0 ? olo[d] : lo[d]) : 0.f;
+        const float h1 = d < ND ? (slot == 0 ? ohi[d] : hi[d]) : 0.f;
+        rec.lo[0][d] = l0;
+        rec.hi[0][d] = h0;
+        rec.lo[1][d] = l1;
+        rec.hi[1][d] = h1;
+      }
+      rec.child[0] = c0;
+      rec.child[1] = c1;
       rec.pad[0] = rec.pad[1] = 0u;
       nodes[node] = rec;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        node_box[(uint64_t)node * 6 + d] = d < ND ? ulo[d] : 0.f;
-        node_box[(uint64_t)node * 6 + 3 + d] = d < ND ? uhi[d] : 0.f;
+      for (int d = 0; d < ND; ++d) {
+        lo[d] = fminf(lo[d], olo[d]);
+        hi[d] = fmaxf(hi[d], ohi[d]);
       }
       if (node == 0) break;
+      me = node;
       node = node_parent[node];
     }
+  }
+}
+
+// Lower bound of each query's Morton code among B's sorted codes, queries
+// listed in Morton order: lanes of a wave hold neighbouring codes, so their
+// binary searches read the same cache lines.
+template <int ND>
+__global__ void StartLeafKernel(const float* __restrict__ q,
+                                const uint32_t* __restrict__ order,
+                                uint32_t n_q, LbvhFrame f,
+                                const uint32_t* __restrict__ bcodes,
+                                uint32_t n_b, uint32_t leaf, uint32_t n_leaf,
+                                uint32_t* __restrict__ start) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_q;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t c = MortonCode32<ND>(q + (uint64_t)order[i] * ND, f);
+    uint32_t lo = 0, hi = n_b;  // first index with bcodes[k] >= c
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (bcodes[mid] < c) lo = mid + 1;
+      else hi = mid;
+    }
+    start[i] = min(lo / leaf, n_leaf - 1u);
   }
 }
 
@@ -246,8 +318,8 @@ __global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
   unsigned long long evals = 0;
   for (uint32_t base = args.pos_begin + wave_base; base < args.pos_end;
        base += lanes_total) {
-    const uint32_t pos = base + lane;
-    const bool active = pos < args.pos_end;
+    const bool active = base + lane < args.pos_end;
+    const uint32_t pos = args.stride > 1u ? (base + lane) * args.stride : base + lane;
     const float cmax2 = args.eb ? LoadCmax2(args.key) : 0.f;
     float best = kFltMax;
     uint32_t best_j = 0xFFFFFFFFu;
@@ -260,9 +332,35 @@ __global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
       float q[3] = {0.f, 0.f, 0.f};
 #pragma unroll
       for (int d = 0; d < ND; ++d) q[d] = args.a[(uint64_t)idx * ND + d];
+      // Exact scan of the leaves around the query's Morton position first:
+      // on clustered data a point within cmax2 is usually there, and the
+      // query ends without a traversal; otherwise the traversal starts with
+      // their minimum as its bound.
+      bool skip = false;
+      if (args.start_leaf != nullptr) {
+        const uint32_t s = args.start_leaf[pos];
+        const uint32_t l0 = s > 0u ? s - 1u : 0u;
+        const uint32_t l1 = min(s + 2u, t.n_leaf);
+        for (uint32_t li = l0; li < l1; ++li) {
+          const uint32_t b1 = min((li + 1u) * t.leaf, t.n);
+          for (uint32_t k = li * t.leaf; k < b1; ++k) {
+            float d2;
+            if constexpr (ND == 2)
+              d2 = D2(q[0], q[1], t.x[k], t.y[k]);
+            else
+              d2 = D2(q[0], q[1], q[2], t.x[k], t.y[k], t.z[k]);
+            evals++;
+            if (d2 < best) best = d2;
+          }
+        }
+        if (args.eb && best < cmax2) {
+          broke = true;
+          skip = true;
+        }
+      }
       int sp = 0;
       uint32_t node = t.n_leaf == 1 ? (0u | kLeafFlag) : 0u;
-      while (true) {
+      while (!skip) {
         // Child ids come from the build; an id out of range would mean a
         // malformed tree, so the query is handed to the exhaustive path
         // instead of reading outside the arrays.
@@ -357,7 +455,7 @@ __global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
 
 int MortonOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
                 const LbvhFrame& f, uint32_t** order_out) {
-  unsigned long long *keys = nullptr, *keys_s = nullptr;
+  uint32_t *keys = nullptr, *keys_s = nullptr;
   uint32_t *vals = nullptr, *order = nullptr;
   HD_TRY(ContextArray(ctx, "lbvh_qkeys", n, &keys));
   HD_TRY(ContextArray(ctx, "lbvh_qkeys_s", n, &keys_s));
@@ -365,12 +463,28 @@ int MortonOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
   HD_TRY(ContextArray(ctx, "lbvh_qorder", n, &order));
   const int blocks = (int)std::min<uint64_t>((n + 255) / 256, 8192);
   if (n_dims == 2)
-    MortonKernel<2><<<blocks, 256, 0, ctx->stream>>>(pts, n, f, keys, vals);
+    MortonKernel32<2><<<blocks, 256, 0, ctx->stream>>>(pts, n, f, keys, vals);
   else
-    MortonKernel<3><<<blocks, 256, 0, ctx->stream>>>(pts, n, f, keys, vals);
+    MortonKernel32<3><<<blocks, 256, 0, ctx->stream>>>(pts, n, f, keys, vals);
   HD_HIP_TRY(hipGetLastError());
-  HD_TRY(SortPairsU64(ctx, keys, keys_s, vals, order, n, n_dims == 2 ? 64 : 63));
+  HD_TRY(SortPairsU32(ctx, keys, keys_s, vals, order, n, 32));
   *order_out = order;
+  return HD_OK;
+}
+
+int StartLeaves(hd_context* ctx, const LbvhIndex& tree, const float* q,
+                const uint32_t* order, uint32_t n, uint32_t* start) {
+  if (n == 0) return HD_OK;
+  const int blocks = (int)std::min<uint64_t>((n + 255) / 256, 8192);
+  if (tree.n_dims == 2)
+    StartLeafKernel<2><<<blocks, 256, 0, ctx->stream>>>(
+        q, order, n, tree.frame, tree.codes, tree.dev.n, tree.dev.leaf,
+        tree.dev.n_leaf, start);
+  else
+    StartLeafKernel<3><<<blocks, 256, 0, ctx->stream>>>(
+        q, order, n, tree.frame, tree.codes, tree.dev.n, tree.dev.leaf,
+        tree.dev.n_leaf, start);
+  HD_HIP_TRY(hipGetLastError());
   return HD_OK;
 }
 
@@ -391,7 +505,7 @@ int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
     f.lo[d] = mbr[d];
     f.scale[d] = (float)(levels / ((double)mbr[nd + d] - (double)mbr[d]));
   }
-  unsigned long long *keys = nullptr, *keys_s = nullptr;
+  uint32_t *keys = nullptr, *keys_s = nullptr;
   uint32_t *vals = nullptr, *vals_s = nullptr;
   HD_TRY(ContextArray(ctx, "lbvh_keys", n, &keys));
   HD_TRY(ContextArray(ctx, "lbvh_keys_s", n, &keys_s));
@@ -399,11 +513,11 @@ int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
   HD_TRY(ContextArray(ctx, "lbvh_vals_s", n, &vals_s));
   const int blocks = (int)std::min<uint64_t>((n + 255) / 256, 8192);
   if (nd == 2)
-    MortonKernel<2><<<blocks, 256, 0, ctx->stream>>>(b, n, f, keys, vals);
+    MortonKernel32<2><<<blocks, 256, 0, ctx->stream>>>(b, n, f, keys, vals);
   else
-    MortonKernel<3><<<blocks, 256, 0, ctx->stream>>>(b, n, f, keys, vals);
+    MortonKernel32<3><<<blocks, 256, 0, ctx->stream>>>(b, n, f, keys, vals);
   HD_HIP_TRY(hipGetLastError());
-  HD_TRY(SortPairsU64(ctx, keys, keys_s, vals, vals_s, n, nd == 2 ? 64 : 63));
+  HD_TRY(SortPairsU32(ctx, keys, keys_s, vals, vals_s, n, 32));
 
   LbvhDev& t = out->dev;
   std::memset(&t, 0, sizeof(t));
@@ -424,15 +538,15 @@ int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
   const uint32_t n_leaf = (n + leaf - 1) / leaf;
   const uint32_t n_int = n_leaf > 1 ? n_leaf - 1 : 1;
   uint32_t *child = nullptr, *node_parent = nullptr, *leaf_parent = nullptr,
-           *arrive = nullptr;
-  float *leaf_box = nullptr, *node_box = nullptr;
+           *arrive = nullptr, *pub = nullptr;
+  float* leaf_box = nullptr;
   LbvhNode* nodes = nullptr;
   HD_TRY(ContextArray(ctx, "lbvh_child", 2 * (uint64_t)n_int, &child));
   HD_TRY(ContextArray(ctx, "lbvh_node_parent", n_int, &node_parent));
   HD_TRY(ContextArray(ctx, "lbvh_leaf_parent", n_leaf, &leaf_parent));
   HD_TRY(ContextArray(ctx, "lbvh_arrive", n_int, &arrive));
   HD_TRY(ContextArray(ctx, "lbvh_leaf_box", 6 * (uint64_t)n_leaf, &leaf_box));
-  HD_TRY(ContextArray(ctx, "lbvh_node_box", 6 * (uint64_t)n_int, &node_box));
+  HD_TRY(ContextArray(ctx, "lbvh_pub", 4 * (uint64_t)nd * n_int, &pub));
   HD_TRY(ContextArray(ctx, "lbvh_nodes", n_int, &nodes));
   HD_TRY(FillU32(ctx, arrive, 0u, n_int));
   if (n_leaf > 1) {
@@ -442,14 +556,19 @@ int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
     HD_HIP_TRY(hipGetLastError());
   }
   const int lb = (int)std::min<uint64_t>((n_leaf + 255) / 256, 8192);
-  if (nd == 2)
-    RefitKernel<2><<<lb, 256, 0, ctx->stream>>>(
-        (int)n_leaf, leaf, n, x, y, z, child, node_parent, leaf_parent, arrive,
-        leaf_box, node_box, nodes);
-  else
-    RefitKernel<3><<<lb, 256, 0, ctx->stream>>>(
-        (int)n_leaf, leaf, n, x, y, z, child, node_parent, leaf_parent, arrive,
-        leaf_box, node_box, nodes);
+  if (nd == 2) {
+    LeafBoxKernel<2><<<lb, 256, 0, ctx->stream>>>((int)n_leaf, leaf, n, x, y,
+                                                  z, leaf_box);
+    RefitKernel<2><<<lb, 256, 0, ctx->stream>>>((int)n_leaf, leaf_box, child,
+                                                node_parent, leaf_parent,
+                                                arrive, pub, nodes);
+  } else {
+    LeafBoxKernel<3><<<lb, 256, 0, ctx->stream>>>((int)n_leaf, leaf, n, x, y,
+                                                  z, leaf_box);
+    RefitKernel<3><<<lb, 256, 0, ctx->stream>>>((int)n_leaf, leaf_box, child,
+                                                node_parent, leaf_parent,
+                                                arrive, pub, nodes);
+  }
   HD_HIP_TRY(hipGetLastError());
   t.nodes = nodes;
   t.x = x;
@@ -461,6 +580,7 @@ int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
   t.leaf = leaf;
   out->n_dims = nd;
   out->frame = f;
+  out->codes = keys_s;
   return HD_OK;
 }
 
@@ -559,21 +679,35 @@ int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
   // Visit the early-break queries in Morton order of A (in B's frame), so
   // lanes of a wave descend the same subtrees: coherent node loads.
   uint32_t* morton_order = nullptr;
+  uint32_t* start_leaf = nullptr;
   if (!(in.flags & HD_FLAG_LEARNED_ORDER)) {
     const int st_ord = StageBegin(ctx, "lbvh_order");
     HD_TRY(MortonOrder(ctx, ND, in.a, n_a, tree.frame, &morton_order));
+    HD_TRY(ContextArray(ctx, "lbvh_start_leaf", n_a, &start_leaf));
+    HD_TRY(StartLeaves(ctx, tree, in.a, morton_order, n_a, start_leaf));
     StageEnd(ctx, st_ord);
   }
 
-  // Seed with exact nearest neighbours of the first points of the order,
-  // then early break for the rest.
+  // Seed with exact nearest neighbours of the first points of the order
+  // (in Morton order: every (n_a / s0)-th point, a spatially stratified
+  // sample whose start leaves give each traversal a tight first bound), then
+  // early break for the rest.
   const uint32_t s0 = std::min<uint32_t>(n_a, 65536u);
   const int st_seed = StageBegin(ctx, "lbvh_seed");
   qa.pos_begin = 0;
   qa.pos_end = s0;
   qa.eb = 0;
   qa.evals = key + 1;
-  HD_TRY(LaunchLbvhQuery(ctx, qa));
+  if (morton_order != nullptr) {
+    LbvhQueryArgs sa = qa;
+    sa.order = morton_order;
+    sa.order_mode = 0;
+    sa.start_leaf = start_leaf;
+    sa.stride = n_a / s0;
+    HD_TRY(LaunchLbvhQuery(ctx, sa));
+  } else {
+    HD_TRY(LaunchLbvhQuery(ctx, qa));
+  }
   StageEnd(ctx, st_seed);
   const int st_q = StageBegin(ctx, "lbvh_query");
   if (morton_order != nullptr) {
@@ -582,6 +716,7 @@ int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
     qa.order = morton_order;
     qa.order_mode = 0;
     qa.pos_begin = 0;
+    qa.start_leaf = start_leaf;
   } else {
     qa.pos_begin = s0;
   }
