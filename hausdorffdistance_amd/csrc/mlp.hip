@@ -16,8 +16,9 @@
 // rises.
 //
 // Kernels: forward+backward with one lane per sample (weights in LDS as
-// FP32 converted from BF16), a per-parameter kernel that reduces the
-// gradient over the batch and applies Adam, and an inference kernel that
+// FP32 converted from BF16), a kernel that reduces the gradient over the
+// batch (one block per 64 parameters, lanes on parameters, waves on
+// samples) and applies Adam, and an inference kernel that
 // runs the 64x64 layer on matrix cores (BF16 operands, FP32 accumulation;
 // the sine activations are rounded to BF16 as MFMA operands).
 #include <hip/hip_bf16.h>
@@ -225,49 +226,85 @@ struct AdamArgs {
   uint32_t* step_ptr;  // advanced by thread 0 after the update
 };
 
-// One thread per parameter: gradient summed over the batch, then Adam.
+// Adam update of parameter p with gradient g (step t = *step_ptr + 1).
+HD_DEV void AdamUpdate(const AdamArgs& a, int p, float g) {
+  if (a.grad_out) a.grad_out[p] = g;
+  if (!a.apply) return;
+  const uint32_t t = *a.step_ptr + 1;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float m = b1 * a.m[p] + (1.f - b1) * g;
+  const float v = b2 * a.v[p] + (1.f - b2) * g * g;
+  a.m[p] = m;
+  a.v[p] = v;
+  const float mh = m / (1.f - powf(b1, (float)t));
+  const float vh = v / (1.f - powf(b2, (float)t));
+  const float w = a.master[p] - a.lr * mh / (sqrtf(vh) + eps);
+  a.master[p] = w;
+  a.wbf[p] = F32ToBf16(w);
+}
+
+constexpr int kGradWaves = 16;  // waves per block of the gradient reduction
+
+// Gradients summed over the batch, then Adam. Each block owns 64
+// parameters that share one per-sample factor (lane l: parameter l):
+//   blocks 0..ND-1:   W1[l][k] = sum_s g1[s][l] x[s][k]   (k = block)
+//   block  ND:        b1[l]    = sum_s g1[s][l]
+//   blocks ND+1..+64: W2[j][l] = sum_s g2[s][j] h1[s][l]  (j = block-ND-1),
+//                     and b2[j] = sum_s g2[s][j]
+//   block  ND+65:     w3[l]    = sum_s go[s] h2[s][l],  and b3 = sum_s go[s]
+// The waves of a block split the batch (wave w: samples w, w + 16, ...);
+// lanes read 64 consecutive floats of a sample row, the shared factor is one
+// broadcast load. Partial sums meet in LDS in a fixed order (deterministic).
 template <int ND>
-__global__ __launch_bounds__(256) void MlpGradAdamKernel(AdamArgs a) {
+__global__ __launch_bounds__(64 * kGradWaves) void MlpGradAdamKernel(AdamArgs a) {
+  __shared__ float red[kGradWaves][kH + 1];
   const Layout L = MakeLayout(ND);
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < L.n) {
-    float g = 0.f;
-    const uint32_t B = a.batch;
-    if (p < L.b1) {  // W1[j][k]
-      const int j = p / ND, k = p % ND;
-      for (uint32_t s = 0; s < B; ++s)
-        g = fmaf(a.g1[(uint64_t)s * kH + j], a.bx[(uint64_t)s * ND + k], g);
-    } else if (p < L.w2) {  // b1[j]
-      const int j = p - L.b1;
-      for (uint32_t s = 0; s < B; ++s) g += a.g1[(uint64_t)s * kH + j];
-    } else if (p < L.b2) {  // W2[j][k]
-      const int j = (p - L.w2) / kH, k = (p - L.w2) % kH;
-      for (uint32_t s = 0; s < B; ++s)
-        g = fmaf(a.g2[(uint64_t)s * kH + j], a.h1[(uint64_t)s * kH + k], g);
-    } else if (p < L.w3) {  // b2[j]
-      const int j = p - L.b2;
-      for (uint32_t s = 0; s < B; ++s) g += a.g2[(uint64_t)s * kH + j];
-    } else if (p < L.b3) {  // w3[k]
-      const int k = p - L.w3;
-      for (uint32_t s = 0; s < B; ++s)
-        g = fmaf(a.go[s], a.h2[(uint64_t)s * kH + k], g);
-    } else {  // b3
-      for (uint32_t s = 0; s < B; ++s) g += a.go[s];
+  const int g = blockIdx.x;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int wave = (int)(threadIdx.x >> 6);
+  const uint32_t B = a.batch;
+  float acc = 0.f, accb = 0.f;  // accb: the group's bias (one value)
+  if (g <= ND) {
+    const int k = g;
+    for (uint32_t s = wave; s < B; s += kGradWaves) {
+      const float f = k < ND ? a.bx[(uint64_t)s * ND + k] : 1.f;
+      acc = fmaf(a.g1[(uint64_t)s * kH + lane], f, acc);
     }
-    if (a.grad_out) a.grad_out[p] = g;
-    if (a.apply) {
-      const uint32_t t = *a.step_ptr + 1;
-      const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-      const float m = b1 * a.m[p] + (1.f - b1) * g;
-      const float v = b2 * a.v[p] + (1.f - b2) * g * g;
-      a.m[p] = m;
-      a.v[p] = v;
-      const float mh = m / (1.f - powf(b1, (float)t));
-      const float vh = v / (1.f - powf(b2, (float)t));
-      const float w = a.master[p] - a.lr * mh / (sqrtf(vh) + eps);
-      a.master[p] = w;
-      a.wbf[p] = F32ToBf16(w);
+  } else if (g <= ND + kH) {
+    const int j = g - ND - 1;
+    for (uint32_t s = wave; s < B; s += kGradWaves) {
+      const float f = a.g2[(uint64_t)s * kH + j];
+      acc = fmaf(f, a.h1[(uint64_t)s * kH + lane], acc);
+      accb += f;
     }
+  } else {
+    for (uint32_t s = wave; s < B; s += kGradWaves) {
+      const float f = a.go[s];
+      acc = fmaf(f, a.h2[(uint64_t)s * kH + lane], acc);
+      accb += f;
+    }
+  }
+  red[wave][lane] = acc;
+  if (lane == 0) red[wave][kH] = accb;
+  __syncthreads();
+  if (wave != 0) return;
+  float sum = 0.f, sumb = 0.f;
+#pragma unroll
+  for (int w = 0; w < kGradWaves; ++w) {
+    sum += red[w][lane];
+    sumb += red[w][kH];
+  }
+  if (g < ND) {
+    AdamUpdate(a, L.w1 + lane * ND + g, sum);
+  } else if (g == ND) {
+    AdamUpdate(a, L.b1 + lane, sum);
+  } else if (g <= ND + kH) {
+    const int j = g - ND - 1;
+    AdamUpdate(a, L.w2 + j * kH + lane, sum);
+    if (lane == 0) AdamUpdate(a, L.b2 + j, sumb);
+  } else {
+    AdamUpdate(a, L.w3 + lane, sum);
+    if (lane == 0) AdamUpdate(a, L.b3, sumb);
   }
 }
 
@@ -531,7 +568,8 @@ int LaunchStep(hd_context* ctx, hd_mlp* m, const TrainBuffers& tb,
   aa.apply = apply;
   aa.lr = lr;
   aa.step_ptr = tb.step;
-  MlpGradAdamKernel<ND><<<(m->n_params + 255) / 256, 256, 0, ctx->stream>>>(aa);
+  // Blocks: ND columns of W1, b1, 64 rows of W2 (+ b2), w3 (+ b3).
+  MlpGradAdamKernel<ND><<<ND + kH + 2, 64 * kGradWaves, 0, ctx->stream>>>(aa);
   HD_HIP_TRY(hipGetLastError());
   if (apply) {
     StepAdvanceKernel<<<1, 1, 0, ctx->stream>>>(tb.step);
