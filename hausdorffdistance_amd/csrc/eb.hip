@@ -524,6 +524,19 @@ HD_DEV float LoadTile(const float* bt, uint64_t tile, uint32_t lane4) {
   return LoadRow(bt, tile << 6, lane4);
 }
 
+// The four tiles of one step through one buffer descriptor (1 KiB of
+// records; the tile offsets go into the instructions' immediate field).
+HD_DEV void LoadStepTiles(const float* bt, uint32_t step, uint32_t lane4,
+                          float* out) {
+  __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(bt + ((uint64_t)Uniform(step) << 8)), (short)0, 1024,
+      0x00020000);
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    out[q] = __uint_as_float(
+        __builtin_amdgcn_raw_buffer_load_b32(r, (int)(lane4 + 256u * q), 0, 0));
+}
+
 // One v_min3_i32 (the compiler would split min(a, min(b, c)) into two
 // instructions to shorten the chain on a; here the chain is on one
 // register per tile and every VALU slot counts against the matrix core).
@@ -635,11 +648,8 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
     float bcur[8], ocur[2][ND];
     {
       const uint32_t st1 = next_step(step);
-  #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        bcur[q] = LoadTile(args.bt, (uint64_t)Uniform(step) * 4u + q, lane4);
-        bcur[4 + q] = LoadTile(args.bt, (uint64_t)Uniform(st1) * 4u + q, lane4);
-      }
+      LoadStepTiles(args.bt, step, lane4, bcur);
+      LoadStepTiles(args.bt, st1, lane4, bcur + 4);
       LoadStep<ND>(args.bx, args.by, args.bz, step, lane, ocur[0]);
       LoadStep<ND>(args.bx, args.by, args.bz, st1, lane, ocur[1]);
     }
@@ -751,23 +761,22 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
       }
       if (live == 0ull) break;
 
-      // Two steps: 8 B tiles against the 4 A tiles (32 MFMAs).
-      const uint32_t s2 = next_step(next_step(step));
-      const uint32_t s3 = next_step(s2);
-      float bnx[8], onx[2][ND];
-  #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        bnx[q] = LoadTile(args.bt, (uint64_t)Uniform(s2) * 4u + q, lane4);
-        bnx[4 + q] = LoadTile(args.bt, (uint64_t)Uniform(s3) * 4u + q, lane4);
-      }
-      LoadStep<ND>(args.bx, args.by, args.bz, s2, lane, onx[0]);
-      LoadStep<ND>(args.bx, args.by, args.bz, s3, lane, onx[1]);
-      // Segmented scans poll the other segments' verdicts every 8 steps.
+      // Segmented scans poll the other segments' verdicts every 8 steps
+      // (issued before the next steps' loads: loads complete in order, so
+      // waiting for the flag does not wait for them).
       const bool poll = split && (gstep & 7u) == 6u;
       uint32_t flag = 0;
       if (poll && ((live >> lane) & 1ull))
         flag = __hip_atomic_load(&args.broken[m_pos], __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT);
+      // Two steps: 8 B tiles against the 4 A tiles (32 MFMAs).
+      const uint32_t s2 = next_step(next_step(step));
+      const uint32_t s3 = next_step(s2);
+      float bnx[8], onx[2][ND];
+      LoadStepTiles(args.bt, s2, lane4, bnx);
+      LoadStepTiles(args.bt, s3, lane4, bnx + 4);
+      LoadStep<ND>(args.bx, args.by, args.bz, s2, lane, onx[0]);
+      LoadStep<ND>(args.bx, args.by, args.bz, s3, lane, onx[1]);
       int32_t sm[4];  // this iteration's approximate minima
       const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
   #pragma unroll
