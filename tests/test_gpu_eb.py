@@ -170,6 +170,21 @@ def test_matrix_core_filter_grouped_round_with_all_points_tied(ctx):
                             "argmax": 0, "nn": 0}
 
 
+def test_matrix_core_filter_grouped_round_3d_tied(ctx):
+    # The same in 3D: a 42^3 lattice (74,088 points, 1158 steps of B) against
+    # itself shifted by half a step in x; every nearest neighbour is at
+    # exactly 2^-8.
+    g = np.arange(42, dtype=np.float32) * np.float32(2.0 ** -7)
+    xx, yy, zz = np.meshgrid(g, g, g, indexing="ij")
+    a = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1).astype(np.float32)
+    b = a + np.array([2.0 ** -8, 0.0, 0.0], np.float32)
+    print("[filter] tied 3D lattice, grouped round", a.shape, flush=True)
+    r, st = _eb_with_stages(ctx, a, b, symmetric=False)
+    assert st.get("eb_mfma_r3", {}).get("points", 0) > 0, sorted(st)
+    assert triple(r.ab) == {"cmax2_bits": int(np.float32(2.0 ** -16).view(np.uint32)),
+                            "argmax": 0, "nn": 0}
+
+
 def test_matrix_core_filter_far_from_origin(ctx):
     # A large common offset: the filter centres coordinates before the
     # matrix products; results must not change.
