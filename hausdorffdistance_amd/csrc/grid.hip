@@ -30,6 +30,7 @@
 #include "hd_device.h"
 #include "hd_grid.h"
 #include "hd_internal.h"
+#include "hd_lbvh.h"
 #include "hd_primitives.h"
 
 namespace hd {
@@ -300,8 +301,10 @@ __global__ __launch_bounds__(kBlock) void GridQueryKernel(GridQueryArgs args) {
   unsigned long long evals = 0;
   for (uint32_t base = args.pos_begin + wave_base; base < args.pos_end;
        base += lanes_total) {
-    const uint32_t pos = base + lane;
-    const bool active = pos < args.pos_end;
+    const uint32_t pos = args.stride > 1u ? (base + lane) * args.stride : base + lane;
+    const bool active = base + lane < args.pos_end &&
+                        !(args.skip_stride > 1u && pos % args.skip_stride == 0u &&
+                          pos / args.skip_stride < args.skip_n);
     const float cmax2 = args.eb ? LoadCmax2(args.key) : 0.f;
     QueryState st{kFltMax, 0xFFFFFFFFu, 0, 0, false, false, false};
     uint32_t idx = 0;
@@ -477,6 +480,9 @@ int LaunchGridQuery(hd_context* ctx, const GridQueryArgs& args) {
 
 namespace {
 
+// Non-empty cells a seed query may look up before it escalates.
+constexpr uint32_t kSeedCells = 1024u;
+
 template <int ND>
 int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
                 hd_directed_result* out) {
@@ -495,12 +501,12 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
   unsigned int* esc_count = nullptr;
   HD_TRY(ContextArray(ctx, "grid_esc_count", 1, &esc_count));
   HD_TRY(FillU32(ctx, esc_count, 0u, 1));
+  // Escalations: queries that looked up too many cells (the hybrid's
+  // max_hit, and every seed query past kSeedCells) finish on the EB rounds.
   uint32_t* esc_idx = nullptr;
   float* esc_min = nullptr;
-  if (max_cells != 0) {
-    HD_TRY(ContextArray(ctx, "grid_esc_idx", n_a, &esc_idx));
-    HD_TRY(ContextArray(ctx, "grid_esc_min", n_a, &esc_min));
-  }
+  HD_TRY(ContextArray(ctx, "grid_esc_idx", n_a, &esc_idx));
+  HD_TRY(ContextArray(ctx, "grid_esc_min", n_a, &esc_min));
 
   GridQueryArgs qa;
   std::memset(&qa, 0, sizeof(qa));
@@ -521,28 +527,57 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
   qa.esc_idx = esc_idx;
   qa.esc_min = esc_min;
 
-  // Seed: exact nearest neighbours of the first points of the order.
+  // Queries in Morton order of A (in B's frame): lanes of a wave look up
+  // neighbouring cells, so their table and point loads share cache lines
+  // (a learned order is kept as given).
   const uint32_t s0 = std::min<uint32_t>(n_a, 65536u);
+  uint32_t stride = 1;
+  if (qa.order_mode != 0) {
+    const int st_ord = StageBegin(ctx, "grid_order");
+    LbvhFrame f;
+    std::memset(&f, 0, sizeof(f));
+    const double levels = ND == 2 ? 4294967040.0 : 2097151.0;
+    f.qmax = (float)levels;
+    for (int d = 0; d < ND; ++d) {
+      f.lo[d] = grid.dev.lo[d];
+      f.scale[d] = (float)(levels / ((double)grid.dev.hi[d] - (double)grid.dev.lo[d]));
+    }
+    uint32_t* mo = nullptr;
+    HD_TRY(MortonOrder(ctx, ND, in.a, n_a, f, &mo));
+    StageEnd(ctx, st_ord);
+    qa.order = mo;
+    qa.order_mode = 0;
+    stride = n_a / s0;
+  }
+
+  // Seed: exact nearest neighbours of s0 points of the order (in Morton
+  // order: every stride-th point, a spatially stratified sample).
   const int st_seed = StageBegin(ctx, "grid_seed");
   qa.pos_begin = 0;
   qa.pos_end = s0;
+  qa.stride = stride;
   qa.eb = 0;
-  // Far seed queries (long runs of empty rings) escalate like the rest.
-  qa.max_cells = max_cells ? std::max<uint32_t>(max_cells, 1024u) : 0u;
+  // Far seed queries (long runs of empty rings) escalate: the seed is a few
+  // waves, so one far query would set its time.
+  qa.max_cells = max_cells ? std::max<uint32_t>(max_cells, kSeedCells) : kSeedCells;
   qa.evals = key + 1;
   HD_TRY(LaunchGridQuery(ctx, qa));
   StageEnd(ctx, st_seed);
-  // Everything else with early break against the running cmax2.
+  // Everything else with early break against the running cmax2 (with a
+  // stratified seed, the whole order minus the seed's positions).
   const int st_q = StageBegin(ctx, "grid_query");
-  qa.pos_begin = s0;
+  qa.pos_begin = stride > 1 ? 0u : s0;
   qa.pos_end = n_a;
+  qa.stride = 1;
+  qa.skip_stride = stride;
+  qa.skip_n = s0;
   qa.eb = 1;
   qa.max_cells = max_cells;
   qa.evals = key + 2;
   HD_TRY(LaunchGridQuery(ctx, qa));
   StageEnd(ctx, st_q);
   uint32_t n_esc = 0;
-  if (max_cells != 0) {
+  {
     // Queries that needed too many cells: exact early-break scans, starting
     // from the partial minimum the grid found.
     void* pinned = nullptr;

@@ -49,6 +49,9 @@ struct GridQueryArgs {
   uint32_t* esc_idx;       // escalations (max_cells exhausted)
   float* esc_min;
   unsigned int* esc_count;
+  uint32_t stride;         // > 1: query k takes position k * stride
+  uint32_t skip_stride;    // > 1: positions k * skip_stride, k < skip_n,
+  uint32_t skip_n;         // are skipped (done by a strided pass)
 };
 
 int BuildGrid(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,

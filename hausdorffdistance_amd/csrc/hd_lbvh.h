@@ -62,6 +62,8 @@ struct LbvhQueryArgs {
   const uint32_t* start_leaf;  // optional, by position: the leaf where the
                                // query's Morton code falls among B's codes
   uint32_t stride;             // > 1: query k takes position k * stride
+  uint32_t skip_stride;        // > 1: positions k * skip_stride, k < skip_n,
+  uint32_t skip_n;             // are skipped (done by a strided pass)
 };
 
 int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,

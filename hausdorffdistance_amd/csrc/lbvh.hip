@@ -318,8 +318,10 @@ __global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
   unsigned long long evals = 0;
   for (uint32_t base = args.pos_begin + wave_base; base < args.pos_end;
        base += lanes_total) {
-    const bool active = base + lane < args.pos_end;
     const uint32_t pos = args.stride > 1u ? (base + lane) * args.stride : base + lane;
+    const bool active = base + lane < args.pos_end &&
+                        !(args.skip_stride > 1u && pos % args.skip_stride == 0u &&
+                          pos / args.skip_stride < args.skip_n);
     const float cmax2 = args.eb ? LoadCmax2(args.key) : 0.f;
     float best = kFltMax;
     uint32_t best_j = 0xFFFFFFFFu;
@@ -711,12 +713,14 @@ int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
   StageEnd(ctx, st_seed);
   const int st_q = StageBegin(ctx, "lbvh_query");
   if (morton_order != nullptr) {
-    // All points, spatially ordered; the seed points are recomputed (their
-    // keys are already in, so this only costs their traversals).
+    // All points, spatially ordered, minus the seed's positions (with
+    // s0 = n_a the seed took them all).
     qa.order = morton_order;
     qa.order_mode = 0;
-    qa.pos_begin = 0;
+    qa.pos_begin = n_a / s0 > 1 ? 0u : s0;
     qa.start_leaf = start_leaf;
+    qa.skip_stride = n_a / s0;
+    qa.skip_n = s0;
   } else {
     qa.pos_begin = s0;
   }
