@@ -28,7 +28,8 @@ using namespace dev;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kP = 8;
+constexpr int kP = 8;      // A points per wave
+constexpr int kSteps = 4;  // B steps (64 points each) per iteration
 
 struct BatchItem {
   uint32_t pair;
@@ -105,12 +106,23 @@ __global__ __launch_bounds__(kBlock) void BatchEbKernel(BatchArgs args) {
 #pragma unroll
     for (int d = 0; d < ND; ++d) rp[d] = 0.f;
     float cmax2 = LoadCmax2(key);
-    float b0, b1, b2 = 0.f;
+    // kSteps consecutive steps of B per iteration (cyclic in the pair's
+    // steps; a pair shorter than that is re-read, which never changes a
+    // minimum), and the break checks once per iteration.
+    auto next_of = [&](uint32_t st) __attribute__((always_inline)) {
+      return st + 1 == n_steps ? 0u : st + 1;
+    };
+    float bc[kSteps][ND];
     {
-      const uint32_t j = (step << 6) + lane;
-      b0 = bx[j];
-      b1 = by[j];
-      if constexpr (ND == 3) b2 = bz[j];
+      uint32_t st = step;
+#pragma unroll
+      for (int s = 0; s < kSteps; ++s) {
+        const uint32_t j = (st << 6) + lane;
+        bc[s][0] = bx[j];
+        bc[s][1] = by[j];
+        if constexpr (ND == 3) bc[s][2] = bz[j];
+        st = next_of(st);
+      }
     }
     while (true) {
 #pragma unroll
@@ -140,28 +152,42 @@ __global__ __launch_bounds__(kBlock) void BatchEbKernel(BatchArgs args) {
         }
       }
       if (live == 0) break;
-      // Prefetch the next step and cmax2 while this step is evaluated.
-      const uint32_t nstep = step + 1 == n_steps ? 0 : step + 1;
-      const uint32_t jn = (nstep << 6) + lane;
-      const float n0 = bx[jn], n1 = by[jn];
-      const float n2 = ND == 3 ? bz[jn] : 0.f;
+      // Prefetch the next steps and cmax2 while these are evaluated.
+      uint32_t nstep = step;
+#pragma unroll
+      for (int s = 0; s < kSteps; ++s) nstep = next_of(nstep);
+      float bn[kSteps][ND];
+      {
+        uint32_t st = nstep;
+#pragma unroll
+        for (int s = 0; s < kSteps; ++s) {
+          const uint32_t j = (st << 6) + lane;
+          bn[s][0] = bx[j];
+          bn[s][1] = by[j];
+          if constexpr (ND == 3) bn[s][2] = bz[j];
+          st = next_of(st);
+        }
+      }
       const uint32_t cm_next = __hip_atomic_load(
           reinterpret_cast<const uint32_t*>(key) + 1, __ATOMIC_RELAXED,
           __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
       for (int q = 0; q < kP; ++q) {
         if (live & (1u << q)) {
-          float d;
-          if constexpr (ND == 2)
-            d = D2(pa[q][0], pa[q][1], b0, b1);
-          else
-            d = D2(pa[q][0], pa[q][1], pa[q][2], b0, b1, b2);
-          lmin[q] = fminf(lmin[q], d);
+#pragma unroll
+          for (int s = 0; s < kSteps; ++s) {
+            float d;
+            if constexpr (ND == 2)
+              d = D2(pa[q][0], pa[q][1], bc[s][0], bc[s][1]);
+            else
+              d = D2(pa[q][0], pa[q][1], pa[q][2], bc[s][0], bc[s][1], bc[s][2]);
+            lmin[q] = fminf(lmin[q], d);
+          }
         }
       }
-      evals += (unsigned long long)__builtin_popcount(live) * 64ull;
+      evals += (unsigned long long)__builtin_popcount(live) * (64ull * kSteps);
       step = nstep;
-      ++gstep;
+      gstep += kSteps;
 #pragma unroll
       for (int q = 0; q < kP; ++q) {
         if (live & (1u << q)) {
@@ -174,9 +200,10 @@ __global__ __launch_bounds__(kBlock) void BatchEbKernel(BatchArgs args) {
           }
         }
       }
-      b0 = n0;
-      b1 = n1;
-      b2 = n2;
+#pragma unroll
+      for (int s = 0; s < kSteps; ++s)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) bc[s][d] = bn[s][d];
       cmax2 = __uint_as_float(Uniform(cm_next));
     }
   }
@@ -270,6 +297,7 @@ struct BatchPlan {
   hipGraphExec_t exec = nullptr;
   BatchResult* h_results = nullptr;  // pinned
   unsigned long long* h_evals = nullptr;
+  uint32_t chunk = 0;  // A points per work item (chosen by trial)
 };
 
 void DestroyBatchPlan(BatchPlan* p) {
@@ -418,17 +446,47 @@ extern "C" int hd_batch_directed_device(hd_context* ctx, int n_dims,
   if (plan == nullptr || plan->sig != sig) {
     if (plan) DestroyBatchPlan(plan);
     ctx->batch_plan = nullptr;
-    plan = new BatchPlan();
-    plan->sig = sig;
-    const uint32_t chunk = 256;  // A points per work item
-    const int rc = n_dims == 2 ? BuildPlan<2>(ctx, d_a, d_offs_a, d_b, d_offs_b,
-                                              n_pairs, chunk, plan)
-                               : BuildPlan<3>(ctx, d_a, d_offs_a, d_b, d_offs_b,
-                                              n_pairs, chunk, plan);
-    if (rc != HD_OK) {
-      DestroyBatchPlan(plan);
-      return rc;
+    auto build = [&](uint32_t chunk, BatchPlan** out_plan) -> int {
+      BatchPlan* p = new BatchPlan();
+      p->sig = sig;
+      const int rc = n_dims == 2 ? BuildPlan<2>(ctx, d_a, d_offs_a, d_b,
+                                                d_offs_b, n_pairs, chunk, p)
+                                 : BuildPlan<3>(ctx, d_a, d_offs_a, d_b,
+                                                d_offs_b, n_pairs, chunk, p);
+      if (rc != HD_OK) {
+        DestroyBatchPlan(p);
+        return rc;
+      }
+      *out_plan = p;
+      return HD_OK;
+    };
+    // Work-item size (A points per item) by trial on the first call of a
+    // batch: small items balance the big pairs across waves, large ones
+    // spend less on claims and slot refills. Each candidate's graph runs
+    // twice, the second run timed; the fastest is rebuilt and kept.
+    static const uint32_t kChunks[] = {128u, 256u, 512u, 1024u};
+    uint32_t best_chunk = 256u;
+    float best_ms = 0.f;
+    for (uint32_t chunk : kChunks) {
+      BatchPlan* trial = nullptr;
+      HD_TRY(build(chunk, &trial));
+      float ms = 0.f;
+      hipError_t e = hipGraphLaunch(trial->exec, ctx->stream);
+      if (e == hipSuccess) e = hipEventRecord(ctx->ev_begin, ctx->stream);
+      if (e == hipSuccess) e = hipGraphLaunch(trial->exec, ctx->stream);
+      if (e == hipSuccess) e = hipEventRecord(ctx->ev_end, ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(ctx->ev_end);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end);
+      DestroyBatchPlan(trial);
+      if (e != hipSuccess)
+        return Fail(HD_EDEVICE, "batch tuning run: %s", hipGetErrorString(e));
+      if (best_ms == 0.f || ms < best_ms) {
+        best_ms = ms;
+        best_chunk = chunk;
+      }
     }
+    HD_TRY(build(best_chunk, &plan));
+    plan->chunk = best_chunk;
     ctx->batch_plan = plan;
   }
   HD_HIP_TRY(hipEventRecord(ctx->ev_begin, ctx->stream));
