@@ -27,3 +27,4 @@ from .api import (  # noqa: F401
     run_hausdorff_distance,
 )
 from . import gen  # noqa: F401
+from .io import load_points, normalize_points, translate_points  # noqa: F401

@@ -99,7 +99,8 @@ class MlpTrainConfigC(C.Structure):
     ]
 
 
-# Every symbol include/hd.h and include/hd_gen.h declare, with its signature.
+# Every symbol include/hd.h, include/hd_gen.h and include/hd_io.h declare,
+# with its signature.
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _U32 = C.POINTER(C.c_uint32)
@@ -156,7 +157,17 @@ SIGNATURES = {
     "hd_gen_batch_points": (
         C.c_int, [C.c_uint64, C.c_uint32, C.c_double, _U32, _U32, _U64, _U64, _F, _F]),
     "hd_gen_subsample": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, _U32]),
+    # hd_io.h
+    "hd_load_points": (
+        C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_uint64, C.c_char_p,
+                  C.POINTER(_F), _U64]),
+    "hd_free_points": (None, [_F]),
+    "hd_normalize_points": (C.c_int, [_F, C.c_uint64, C.c_int]),
+    "hd_translate_points": (C.c_int, [_F, C.c_uint64, C.c_int, C.c_int, C.c_double]),
 }
+
+# Input types of hd_load_points (include/hd_io.h).
+HD_INPUT_WKT, HD_INPUT_OFF, HD_INPUT_PLY, HD_INPUT_BIN = 0, 1, 2, 3
 
 _lib = None
 

@@ -1,13 +1,13 @@
 // hd_exec: command-line driver over the C ABI, with the reference's flag
 // names (src/flags.cc:3-57, parsed into RunConfig at src/main.cpp:47-132).
-// Inputs: the reference's point cache format (".bin": size_t n, then n
-// packed points; src/loaders/loader.h:150-184), OFF vertex lists
-// (loader.h:105-147, numbers read as one stream, n_dims per point), or a
-// named synthetic workload (include/hd_gen.h). Preprocessing -normalize and
-// -translate follow src/loaders/translate_points.h:42-80.
+// Inputs (include/hd_io.h): WKT, OFF and PLY files, the reference's point
+// cache (".bin": size_t n, then n packed points; -serialize <dir> caches
+// parsed WKT/OFF there, src/loaders/loader.h:176-234), or a named synthetic
+// workload (include/hd_gen.h). Preprocessing -normalize and -translate
+// follow src/loaders/translate_points.h:42-80.
 //
-//   hd_exec -input1 a.bin -input2 b.bin -input_type bin -n_dims 2 \
-//           -variant eb -execution gpu -repeat 5 -json out.json
+//   hd_exec -input1 a.wkt -input2 b.wkt -input_type wkt -n_dims 2 \
+//           -variant eb -execution gpu -repeat 5 -serialize cache -json out.json
 //   hd_exec -input_type synthetic -input1 gmm_10m_2d -n_dims 2 -variant hybrid
 #include <algorithm>
 #include <cctype>
@@ -22,6 +22,7 @@
 
 #include "hd.h"
 #include "hd_gen.h"
+#include "hd_io.h"
 
 namespace {
 
@@ -83,37 +84,18 @@ bool Parse(int argc, char** argv, Flags* f) {
   return true;
 }
 
-bool LoadBin(const std::string& path, int nd, long limit,
-             std::vector<float>* out) {
-  std::ifstream ifs(path, std::ios::binary);
-  if (!ifs) return false;
+// A file through hd_load_points (host memory, AoS).
+bool LoadFile(const std::string& path, int input_type, int nd, long limit,
+              const std::string& serialize, std::vector<float>* out) {
+  float* p = nullptr;
   uint64_t n = 0;
-  ifs.read(reinterpret_cast<char*>(&n), sizeof(n));
-  if (!ifs) return false;
-  n = std::min<uint64_t>(n, (uint64_t)limit);
-  out->resize(n * nd);
-  ifs.read(reinterpret_cast<char*>(out->data()), sizeof(float) * n * nd);
-  return (bool)ifs;
-}
-
-bool LoadOff(const std::string& path, int nd, long limit,
-             std::vector<float>* out) {
-  std::ifstream ifs(path);
-  if (!ifs) return false;
-  std::string line;
-  std::getline(ifs, line);
-  if (line.substr(0, 3) != "OFF") return false;
-  line = line.substr(3);
-  if (line.find_first_not_of(" \t\r") == std::string::npos)
-    std::getline(ifs, line);
-  std::istringstream hs(line);
-  uint64_t nv = 0, nf = 0, ne = 0;
-  hs >> nv >> nf >> ne;
-  if (nv == 0) return false;
-  nv = std::min<uint64_t>(nv, (uint64_t)limit);
-  out->resize(nv * nd);
-  for (uint64_t i = 0; i < nv * nd; ++i)
-    if (!(ifs >> (*out)[i])) return false;
+  if (hd_load_points(path.c_str(), input_type, nd, (uint64_t)limit,
+                     serialize.c_str(), &p, &n) != HD_OK) {
+    fprintf(stderr, "%s\n", hd_last_error());
+    return false;
+  }
+  out->assign(p, p + n * nd);
+  hd_free_points(p);
   return true;
 }
 
@@ -148,47 +130,17 @@ bool Synthetic(const std::string& name, bool second, int nd, long limit,
   return false;
 }
 
-// src/loaders/translate_points.h:42-63
-void Normalize(std::vector<float>& p, int nd) {
-  const size_t n = p.size() / nd;
-  if (n == 0) return;
-  float lo[3], hi[3];
-  for (int d = 0; d < nd; ++d) lo[d] = hi[d] = p[d];
-  for (size_t i = 0; i < n; ++i)
-    for (int d = 0; d < nd; ++d) {
-      lo[d] = std::min(lo[d], p[i * nd + d]);
-      hi[d] = std::max(hi[d], p[i * nd + d]);
-    }
-  float ext = 0;
-  for (int d = 0; d < nd; ++d) ext = std::max(ext, hi[d] - lo[d]);
-  if (ext == 0) ext = 1;
-  for (size_t i = 0; i < n; ++i)
-    for (int d = 0; d < nd; ++d) p[i * nd + d] = (p[i * nd + d] - lo[d]) / ext;
-}
-
-// src/loaders/translate_points.h:65-80 (dimension 0).
-void Translate(std::vector<float>& p, int nd, double ratio) {
-  const size_t n = p.size() / nd;
-  if (n == 0) return;
-  float lo = p[0], hi = p[0];
-  for (size_t i = 0; i < n; ++i) {
-    lo = std::min(lo, p[i * nd]);
-    hi = std::max(hi, p[i * nd]);
-  }
-  for (size_t i = 0; i < n; ++i) p[i * nd] += (float)((hi - lo) * ratio);
-}
-
 }  // namespace
 
 int main(int argc, char** argv) {
   Flags f;
   if (argc == 1 || !Parse(argc, argv, &f)) {
     fprintf(stderr,
-            "Usage: hd_exec -input1 A -input2 B -input_type bin|off|synthetic "
+            "Usage: hd_exec -input1 A -input2 B -input_type wkt|off|ply|bin|synthetic "
             "-n_dims 2|3 [-variant eb|grid|rt|lbvh|hybrid] [-execution gpu] "
             "[-repeat 5] [-seed 0] [-check] [-symmetric] [-learned_order] "
             "[-n_points_cell 8] [-max_hit 256] [-auto_tune] [-limit N] "
-            "[-normalize] [-translate r] [-json out.json]\n");
+            "[-normalize] [-translate r] [-serialize dir] [-json out.json]\n");
     return 1;
   }
   hd_run_config cfg;
@@ -232,12 +184,15 @@ int main(int argc, char** argv) {
   const long limit = f.Int("limit", 0x7FFFFFFFL);
   const std::string type = f.Get("input_type", "");
   const std::string in1 = f.Get("input1", ""), in2 = f.Get("input2", "");
+  const std::string serialize = f.Get("serialize", "");
   std::vector<float> a, b;
   bool ok = false;
-  if (type == "bin") {
-    ok = LoadBin(in1, cfg.n_dims, limit, &a) && LoadBin(in2, cfg.n_dims, limit, &b);
-  } else if (type == "off") {
-    ok = LoadOff(in1, cfg.n_dims, limit, &a) && LoadOff(in2, cfg.n_dims, limit, &b);
+  const int file_type = type == "wkt" ? HD_INPUT_WKT : type == "off" ? HD_INPUT_OFF
+                        : type == "ply" ? HD_INPUT_PLY : type == "bin" ? HD_INPUT_BIN
+                        : -1;
+  if (file_type >= 0) {
+    ok = LoadFile(in1, file_type, cfg.n_dims, limit, serialize, &a) &&
+         LoadFile(in2, file_type, cfg.n_dims, limit, serialize, &b);
   } else if (type == "synthetic") {
     const std::string n2 = in2.empty() ? in1 : in2;
     ok = Synthetic(in1, false, cfg.n_dims, limit, &a) &&
@@ -252,11 +207,11 @@ int main(int argc, char** argv) {
     return 1;
   }
   if (f.Bool("normalize", false)) {
-    Normalize(a, cfg.n_dims);
-    Normalize(b, cfg.n_dims);
+    hd_normalize_points(a.data(), a.size() / cfg.n_dims, cfg.n_dims);
+    hd_normalize_points(b.data(), b.size() / cfg.n_dims, cfg.n_dims);
   }
   const double tr = f.Dbl("translate", 0.0);
-  if (tr != 0) Translate(b, cfg.n_dims, tr);
+  if (tr != 0) hd_translate_points(b.data(), b.size() / cfg.n_dims, cfg.n_dims, 0, tr);
 
   const uint64_t na = a.size() / cfg.n_dims, nb = b.size() / cfg.n_dims;
   printf("Points A: %llu Points B: %llu\n", (unsigned long long)na,
