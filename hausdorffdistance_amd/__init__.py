@@ -26,5 +26,5 @@ from .api import (  # noqa: F401
     nn_device,
     run_hausdorff_distance,
 )
-from . import gen  # noqa: F401
+from . import features, gen  # noqa: F401
 from .io import load_points, normalize_points, translate_points  # noqa: F401

@@ -52,7 +52,7 @@ class RunConfigC(C.Structure):
         ("max_hit", C.c_uint32),
         ("eb_only_threshold", C.c_uint32),
         ("auto_tune", C.c_int32),
-        ("reserved", C.c_int32),
+        ("stats_n_points_cell", C.c_int32),
         ("model", C.c_void_p),
     ]
 
@@ -140,6 +140,9 @@ SIGNATURES = {
     "hd_mlp_get_params": (C.c_int, [_P, _F, _F, _F]),
     "hd_mlp_loss_grad_device": (
         C.c_int, [_P, C.c_int, _F, _F, _F, C.c_uint32, _F, _F]),
+    "hd_stats_gini": (C.c_int, [_U64, C.c_uint64, C.POINTER(C.c_double)]),
+    "hd_stats_percentiles": (C.c_int, [_U64, C.c_uint64, C.c_char_p, C.c_uint64]),
+    "hd_mbr_hd_bounds": (C.c_int, [_F, _F, C.c_int, _F, _F]),
     "hd_context_set_profiling": (C.c_int, [_P, C.c_int]),
     "hd_context_stage_times": (C.c_int, [_P, C.c_char_p, C.c_uint64, C.c_int]),
     "hd_device_count": (C.c_int, []),

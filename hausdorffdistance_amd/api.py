@@ -76,6 +76,7 @@ class RunConfig:
     max_hit: int = 256
     eb_only_threshold: int = 0
     auto_tune: bool = False
+    stats_n_points_cell: int = 10  # > 0: input statistics in the stats JSON
 
     def to_c(self) -> _lib.RunConfigC:
         c = _lib.RunConfigC()
@@ -101,6 +102,7 @@ class RunConfig:
         c.max_hit = int(self.max_hit)
         c.eb_only_threshold = int(self.eb_only_threshold)
         c.auto_tune = 1 if self.auto_tune else 0
+        c.stats_n_points_cell = int(self.stats_n_points_cell)
         return c
 
 

@@ -263,6 +263,7 @@ void hd_run_config_init(hd_run_config* cfg) {
   cfg->max_hit = 256;
   cfg->eb_only_threshold = 0;
   cfg->auto_tune = 0;
+  cfg->stats_n_points_cell = 10;
   cfg->model = nullptr;
 }
 
@@ -477,6 +478,18 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
   std::swap(ba.mbr_a, ba.mbr_b);
   const bool symmetric = (cfg->flags & HD_FLAG_SYMMETRIC) != 0;
 
+  // MBR bounds of the directed HD (hausdorff_distance_hybrid.h:95-106),
+  // written into every repeat.
+  float hd_lb = 0.f, hd_ub = 0.f;
+  {
+    float lo_a[3], hi_a[3], lo_b[3], hi_b[3], lb2, ub2;
+    NonEmptyMbr(nd, mbr_a, lo_a, hi_a);
+    NonEmptyMbr(nd, mbr_b, lo_b, hi_b);
+    HdBounds2(nd, lo_a, hi_a, lo_b, hi_b, &lb2, &ub2);
+    hd_lb = std::sqrt(lb2);
+    hd_ub = std::sqrt(ub2);
+  }
+
   std::vector<double> times;
   std::ostringstream reps;
   for (int r = 0; r < cfg->repeat; ++r) {
@@ -497,7 +510,9 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
          << "\", \"Execution\": \"GPU\", \"ReportedTime\": " << Num(t)
          << ", \"ComparedPairs\": " << (rab.evals + rba.evals)
          << ", \"CMax2Bits\": " << rab.cmax2_bits
-         << ", \"ArgMax\": " << rab.argmax << "}";
+         << ", \"ArgMax\": " << rab.argmax
+         << ", \"HDLowerBound\": " << Num(hd_lb)
+         << ", \"HDUpperBound\": " << Num(hd_ub) << "}";
   }
   // Result.
   out->n_repeats = (uint32_t)cfg->repeat;
@@ -542,21 +557,32 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
   if (stats_json && stats_cap > 0) {
     hipDeviceProp_t prop;
     HD_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
-    auto mbr_json = [&](const float* m) {
-      std::ostringstream o;
-      o << "[";
-      for (int d = 0; d < nd; ++d)
-        o << (d ? ", " : "") << "{\"Lower\": " << Num(m[d])
-          << ", \"Upper\": " << Num(m[nd + d]) << "}";
-      o << "]";
-      return o.str();
-    };
+    // Input statistics (run_hausdorff_distance.cu:151-204): per file and for
+    // both files together; the top-level density is over the union of the
+    // two non-empty MBRs.
+    std::string files[2], both;
+    float mbr_ab[6], lo_a[3], hi_a[3], lo_b[3], hi_b[3], lo[3], hi[3];
+    NonEmptyMbr(nd, mbr_a, lo_a, hi_a);
+    NonEmptyMbr(nd, mbr_b, lo_b, hi_b);
+    for (int d = 0; d < nd; ++d) {
+      mbr_ab[d] = std::min(mbr_a[d], mbr_b[d]);
+      mbr_ab[nd + d] = std::max(mbr_a[nd + d], mbr_b[nd + d]);
+      lo[d] = std::min(lo_a[d], lo_b[d]);
+      hi[d] = std::max(hi_a[d], hi_b[d]);
+    }
+    const float density = (float)(n_a + n_b) / MbrVolume(nd, lo, hi);
+    const StatsInput sets[2] = {{d_a, n_a}, {d_b, n_b}};
+    const uint32_t spc = (uint32_t)std::max(cfg->stats_n_points_cell, 0);
+    HD_TRY(InputStatsJson(c, nd, &sets[0], 1, mbr_a, spc, nullptr, &files[0]));
+    HD_TRY(InputStatsJson(c, nd, &sets[1], 1, mbr_b, spc, nullptr, &files[1]));
+    HD_TRY(InputStatsJson(c, nd, sets, 2, mbr_ab, spc, &density, &both));
+    both.pop_back();  // the union's object, extended below
     std::ostringstream js;
     js << "{\"GPU\": {\"Device\": " << c->device << ", \"name\": \""
-       << JsonEscape(prop.name) << "\"}, \"Input\": {\"Files\": [{"
-       << "\"NumPoints\": " << n_a << ", \"MBR\": " << mbr_json(mbr_a)
-       << "}, {\"NumPoints\": " << n_b << ", \"MBR\": " << mbr_json(mbr_b)
-       << "}], \"NumDims\": " << nd << ", \"Type\": \"Float\"}"
+       << JsonEscape(prop.name) << "\"}, \"Input\": " << both
+       << ", \"Files\": [" << files[0]
+       << ", " << files[1] << "], \"NumDims\": " << nd
+       << ", \"Type\": \"Float\"}"
        << ", \"Running\": {\"Seed\": " << cfg->seed
        << ", \"SampleRate\": " << Num(cfg->sample_rate)
        << ", \"NumPointsPerCell\": " << cfg->n_points_cell

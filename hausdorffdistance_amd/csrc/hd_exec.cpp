@@ -140,6 +140,7 @@ int main(int argc, char** argv) {
             "-n_dims 2|3 [-variant eb|grid|rt|lbvh|hybrid] [-execution gpu] "
             "[-repeat 5] [-seed 0] [-check] [-symmetric] [-learned_order] "
             "[-n_points_cell 8] [-max_hit 256] [-auto_tune] [-limit N] "
+            "[-stats_n_points_cell 10] "
             "[-normalize] [-translate r] [-serialize dir] [-json out.json]\n");
     return 1;
   }
@@ -176,6 +177,7 @@ int main(int argc, char** argv) {
   cfg.max_hit = (uint32_t)f.Int("max_hit", 256);
   cfg.eb_only_threshold = (uint32_t)f.Int("eb_only_threshold", 0);
   cfg.auto_tune = f.Bool("auto_tune", false) ? 1 : 0;
+  cfg.stats_n_points_cell = (int32_t)f.Int("stats_n_points_cell", 10);
   if (f.Bool("check", false)) cfg.flags |= HD_FLAG_CHECK;
   if (f.Bool("symmetric", false)) cfg.flags |= HD_FLAG_SYMMETRIC;
   if (f.Bool("learned_order", false)) cfg.flags |= HD_FLAG_LEARNED_ORDER;
@@ -238,8 +240,25 @@ int main(int argc, char** argv) {
     printf("HausdorffDistance is %s\n", res.check_pass ? "checked" : "WRONG");
   const std::string json = f.Get("json", "");
   if (!json.empty()) {
+    // The loading options the library does not see go into "Input"
+    // (run_hausdorff_distance.cu:196-199).
+    std::string js = stats.data();
+    const std::string key = "\"Input\": {";
+    const size_t at = js.find(key);
+    if (at != std::string::npos) {
+      std::string esc;
+      for (char ch : serialize) {
+        if (ch == '"' || ch == '\\') esc += '\\';
+        esc += ch;
+      }
+      char extra[128];
+      snprintf(extra, sizeof(extra), "\"Normalize\": %s, \"Translate\": %.17g, ",
+               f.Bool("normalize", false) ? "true" : "false", tr);
+      js.insert(at + key.size(), std::string(extra) + "\"SerializationPrefix\": \"" +
+                                     esc + "\", \"Limit\": " + std::to_string(limit) + ", ");
+    }
     std::ofstream ofs(json);
-    ofs << stats.data() << "\n";
+    ofs << js << "\n";
   }
   hd_context_destroy(ctx);
   return res.check_pass == 0 ? 2 : 0;

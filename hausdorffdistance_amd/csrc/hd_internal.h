@@ -245,6 +245,28 @@ int FinishDirected(hd_context* ctx, int n_dims, const float* a,
                    const unsigned long long* evals_dev, int n_counters,
                    hd_directed_result* out);
 
+// Input statistics of the stats JSON (stats.hip): one point set on the
+// device, AoS.
+struct StatsInput {
+  const float* pts;
+  uint64_t n;
+};
+// {"NumPoints", "MBR", "Density", "Grid", "StatsGridNumPointsPerCell"} of the
+// sets taken together; `mbr` is their raw MBR (lower[nd], upper[nd]). With
+// n_points_cell 0 only NumPoints, MBR and Density; a non-null `density`
+// replaces the sets' own.
+int InputStatsJson(hd_context* ctx, int n_dims, const StatsInput* sets,
+                   int n_sets, const float* mbr, uint32_t n_points_cell,
+                   const float* density, std::string* json);
+// Non-empty MBR rule of the stats (a zero extent becomes 1e-5) and the FP32
+// volume.
+void NonEmptyMbr(int n_dims, const float* mbr, float* lo, float* hi);
+float MbrVolume(int n_dims, const float* lo, const float* hi);
+// Squared lower and upper bounds of the directed HD A -> B from the MBRs
+// (bounds.cpp).
+void HdBounds2(int n_dims, const float* lo_a, const float* hi_a,
+               const float* lo_b, const float* hi_b, float* lb2, float* ub2);
+
 }  // namespace hd
 
 #endif  // HD_INTERNAL_H

@@ -83,7 +83,11 @@ typedef struct hd_run_config {
   uint32_t eb_only_threshold; /* hybrid: EB only if sampled p80 iterations
                               fall below this (-eb_only_threshold, 0) */
   int32_t auto_tune;       /* 1: parameters from the autotuner (-auto_tune) */
-  int32_t reserved;
+  int32_t stats_n_points_cell; /* points per cell of the input-statistics
+                              grid written to the stats JSON ("Input":
+                              MBR, density, cell counts, Gini, percentiles;
+                              run_hausdorff_distance.cu:151-216); 0 = none
+                              (-stats_n_points_cell, 10) */
   const void* model;       /* optional hd_mlp* for HD_FLAG_LEARNED_ORDER */
 } hd_run_config;
 
@@ -218,6 +222,29 @@ int hd_mlp_get_params(const hd_mlp* model, float* params, float* norm_lower,
 int hd_mlp_loss_grad_device(hd_context* ctx, int n_dims, const float* params,
                             const float* x, const float* y, uint32_t batch,
                             float* loss, float* grad);
+
+/* Host-side pieces of the input statistics (stats_n_points_cell), as test
+ * hooks: for the multiset whose value v occurs hist[v] times (the number of
+ * grid cells holding v points), the Gini index
+ * sum_i (2i - n + 1) x_i / (n sum x) over the sorted values
+ * (src/running_stats.h:32-57, here in FP64), and the percentile list the
+ * reference writes from its HdrHistogram dump (running_stats.h:106-140,
+ * uniform_grid.h:675-692): [{"value", "percentile", "count"}, ...] at the
+ * levels 0, 1/6, 1/3, 1/2, 7/12, ... (three per halving of the distance to
+ * 1, rounded to 6 decimals), then 1. As in a histogram made for values
+ * 1..(number of cells) with 3 significant figures, values from
+ * 2048 * 2^k above that number are left out and values from 2048 up are
+ * reported as the top of their sub-bucket. */
+int hd_stats_gini(const uint64_t* hist, uint64_t n_values, double* out);
+/* Lower and upper bounds of the directed HD A -> B from the two MBRs
+ * (lower[n_dims], upper[n_dims] each, made non-empty as the stats do), as the
+ * reference's hybrid computes them (src/geoms/hd_bounds.h:111-160 over the
+ * MBRs' faces; hausdorff_distance_hybrid.h:95-106) and writes them into
+ * each repeat of the stats JSON ("HDLowerBound", "HDUpperBound"). */
+int hd_mbr_hd_bounds(const float* mbr_a, const float* mbr_b, int n_dims,
+                     float* lower, float* upper);
+int hd_stats_percentiles(const uint64_t* hist, uint64_t n_values, char* json,
+                         uint64_t cap);
 
 /* Per-stage device timing. With profiling on, each pipeline stage (kernel or
  * library call) is bracketed by HIP events on the context stream;

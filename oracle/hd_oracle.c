@@ -396,7 +396,7 @@ void orc_mbr(const float* p, uint64_t n, int n_dims, float* out) {
       if (v > out[n_dims + d]) out[n_dims + d] = v;
     }
   for (int d = 0; d < n_dims; ++d)
-    if (out[n_dims + d] - out[d] == 0) out[n_dims + d] = out[d] + 1e-5f;
+    if (out[n_dims + d] - out[d] == 0) out[n_dims + d] = (float)((double)out[d] + 1e-5);
 }
 
 /* FP64 forward/backward of the ordering MLP (no reference counterpart: the
