@@ -6,11 +6,19 @@
 // workload (include/hd_gen.h). Preprocessing -normalize and -translate
 // follow src/loaders/translate_points.h:42-80.
 //
+// -vary_params runs the hybrid once per combination of -sample_rate_list,
+// -eb_only_threshold_list, -max_hit_list and -n_points_cell_list and writes
+// one JSON per combination (src/autotune_hausdorff_distance.cu:181-296).
+//
 //   hd_exec -input1 a.wkt -input2 b.wkt -input_type wkt -n_dims 2 \
 //           -variant eb -execution gpu -repeat 5 -serialize cache -json out.json
 //   hd_exec -input_type synthetic -input1 gmm_10m_2d -n_dims 2 -variant hybrid
+//   hd_exec ... -vary_params -n_points_cell_list 1,2,4,8,16 -json logs/pair
+#include <unistd.h>
+
 #include <algorithm>
 #include <cctype>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,6 +27,7 @@
 #include <sstream>
 #include <string>
 #include <vector>
+#include <ctime>
 
 #include "hd.h"
 #include "hd_gen.h"
@@ -50,7 +59,7 @@ struct Flags {
 };
 
 const char* kBoolFlags[] = {"check", "normalize", "symmetric", "learned_order",
-                            "auto_tune", "overwrite", "shuffle"};
+                            "auto_tune", "overwrite", "shuffle", "vary_params"};
 
 bool IsBool(const std::string& k) {
   for (const char* b : kBoolFlags)
@@ -130,6 +139,130 @@ bool Synthetic(const std::string& name, bool second, int nd, long limit,
   return false;
 }
 
+template <typename T>
+std::vector<T> SplitList(const std::string& text) {
+  std::vector<T> out;
+  std::stringstream ss(text);
+  std::string item;
+  while (std::getline(ss, item, ','))
+    if (!item.empty()) out.push_back((T)std::strtod(item.c_str(), nullptr));
+  return out;
+}
+
+std::string JsonString(const std::string& v) {
+  std::string esc;
+  for (char ch : v) {
+    if (ch == '"' || ch == '\\') esc += '\\';
+    esc += ch;
+  }
+  return "\"" + esc + "\"";
+}
+
+// The loading options and file paths the library does not see go into the
+// stats JSON's "Input" (run_hausdorff_distance.cu:196-199; the paths and
+// the time stamp as autotune_hausdorff_distance.cu:96-99,225).
+std::string WithInputOptions(std::string js, const Flags& f, long limit,
+                             double translate, const std::string& date) {
+  const std::string key = "\"Input\": {";
+  const size_t at = js.find(key);
+  if (at == std::string::npos) return js;
+  char extra[160];
+  snprintf(extra, sizeof(extra), "\"Normalize\": %s, \"Translate\": %.17g, ",
+           f.Bool("normalize", false) ? "true" : "false", translate);
+  js.insert(at + key.size(), std::string(extra) + "\"SerializationPrefix\": " +
+                                 JsonString(f.Get("serialize", "")) +
+                                 ", \"Limit\": " + std::to_string(limit) + ", ");
+  const std::string files = "\"Files\": [{";
+  const size_t fa = js.find(files, at);
+  if (fa != std::string::npos) {
+    js.insert(fa + files.size(), "\"Path\": " + JsonString(f.Get("input1", "")) + ", ");
+    const size_t fb = js.find("}, {\"NumPoints\"", fa);
+    if (fb != std::string::npos)
+      js.insert(fb + 4, "\"Path\": " + JsonString(f.Get("input2", f.Get("input1", ""))) + ", ");
+  }
+  if (!date.empty()) js.insert(1, "\"DateTime\": " + JsonString(date) + ", ");
+  return js;
+}
+
+std::string NowString() {
+  const std::time_t t = std::time(nullptr);
+  std::tm tm;
+  localtime_r(&t, &tm);
+  char buf[32];
+  std::strftime(buf, sizeof(buf), "%Y-%m-%d %H:%M:%S", &tm);
+  return buf;
+}
+
+// The parameter sweep (autotune_hausdorff_distance.cu:181-296): the hybrid
+// for every combination, one JSON per combination named
+// <json>_sample_rate_%.6f_eb_only_threshold_%u_max_hit_%u_n_points_cell_%u.json,
+// existing files skipped unless -overwrite. Sets B of at least 500k points
+// sweep only eb_only_threshold 0 (the reference's sample threshold).
+int VaryParams(hd_context* ctx, hd_run_config cfg, const Flags& f,
+               const std::vector<float>& a, const std::vector<float>& b,
+               long limit, double translate) {
+  const uint64_t na = a.size() / cfg.n_dims, nb = b.size() / cfg.n_dims;
+  const auto rates = SplitList<double>(f.Get("sample_rate_list", "0.01"));
+  auto thresholds = SplitList<long>(f.Get("eb_only_threshold_list", "0"));
+  const auto hits = SplitList<long>(f.Get("max_hit_list", "256"));
+  const auto cells = SplitList<long>(f.Get("n_points_cell_list", "8"));
+  if (nb >= 500000) thresholds.assign(1, 0);
+  const std::string json = f.Get("json", "");
+  const bool overwrite = f.Bool("overwrite", false);
+  cfg.variant = HD_VARIANT_HYBRID;
+  cfg.flags &= ~HD_FLAG_CHECK;
+  const size_t total = rates.size() * thresholds.size() * hits.size() * cells.size();
+  size_t done = 0, skipped = 0;
+  double best = -1;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<char> stats(1 << 20);
+  for (double rate : rates)
+    for (long thr : thresholds)
+      for (long hit : hits)
+        for (long npc : cells) {
+          if (npc <= 0 || rate > 1 || rate < 0) {
+            fprintf(stderr, "bad sweep point: sample_rate %g, n_points_cell %ld\n", rate, npc);
+            return 1;
+          }
+          char path[4096];
+          snprintf(path, sizeof(path),
+                   "%s_sample_rate_%.6f_eb_only_threshold_%u_max_hit_%u_n_points_cell_%u.json",
+                   json.c_str(), rate, (unsigned)thr, (unsigned)hit, (unsigned)npc);
+          const bool exists = !json.empty() && access(path, R_OK) == 0;
+          if (exists && !overwrite) {
+            printf("Skip %s\n", path);
+            ++skipped;
+            continue;
+          }
+          cfg.sample_rate = (float)rate;
+          cfg.eb_only_threshold = (uint32_t)thr;
+          cfg.max_hit = (uint32_t)hit;
+          cfg.n_points_cell = (uint32_t)npc;
+          const std::string date = NowString();
+          hd_result res;
+          const int rc = hd_run_hausdorff_distance(ctx, &cfg, a.data(), na, b.data(), nb,
+                                                   &res, stats.data(), stats.size());
+          if (rc != HD_OK) {
+            fprintf(stderr, "error %d: %s\n", rc, hd_last_error());
+            return 1;
+          }
+          best = best < 0 ? res.avg_time_ms : std::min(best, res.avg_time_ms);
+          printf("Sample rate %g EB only threshold %ld Max hit %ld Points/cell %ld: "
+                 "Avg Running Time %.2f ms\n", rate, thr, hit, npc, res.avg_time_ms);
+          if (!json.empty()) {
+            std::ofstream ofs(path);
+            ofs << WithInputOptions(stats.data(), f, limit, translate, date) << "\n";
+          }
+          ++done;
+          const double el = std::chrono::duration<double>(
+                                std::chrono::steady_clock::now() - t0).count();
+          printf("Progress %.2f %% Remaining Time %.2f s Best Performance %.2f ms\n",
+                 100.0 * (done + skipped) / total, el / done * (total - done - skipped), best);
+          fflush(stdout);
+        }
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -140,7 +273,9 @@ int main(int argc, char** argv) {
             "-n_dims 2|3 [-variant eb|grid|rt|lbvh|hybrid] [-execution gpu] "
             "[-repeat 5] [-seed 0] [-check] [-symmetric] [-learned_order] "
             "[-n_points_cell 8] [-max_hit 256] [-auto_tune] [-limit N] "
-            "[-stats_n_points_cell 10] "
+            "[-stats_n_points_cell 10] [-vary_params -sample_rate_list r,.. "
+            "-eb_only_threshold_list n,.. -max_hit_list n,.. -n_points_cell_list n,.. "
+            "[-overwrite]] "
             "[-normalize] [-translate r] [-serialize dir] [-json out.json]\n");
     return 1;
   }
@@ -223,6 +358,11 @@ int main(int argc, char** argv) {
     fprintf(stderr, "%s\n", hd_last_error());
     return 1;
   }
+  if (f.Bool("vary_params", false)) {
+    const int vrc = VaryParams(ctx, cfg, f, a, b, limit, tr);
+    hd_context_destroy(ctx);
+    return vrc;
+  }
   hd_result res;
   std::vector<char> stats(1 << 20);
   int rc = hd_run_hausdorff_distance(ctx, &cfg, a.data(), na, b.data(), nb,
@@ -240,25 +380,8 @@ int main(int argc, char** argv) {
     printf("HausdorffDistance is %s\n", res.check_pass ? "checked" : "WRONG");
   const std::string json = f.Get("json", "");
   if (!json.empty()) {
-    // The loading options the library does not see go into "Input"
-    // (run_hausdorff_distance.cu:196-199).
-    std::string js = stats.data();
-    const std::string key = "\"Input\": {";
-    const size_t at = js.find(key);
-    if (at != std::string::npos) {
-      std::string esc;
-      for (char ch : serialize) {
-        if (ch == '"' || ch == '\\') esc += '\\';
-        esc += ch;
-      }
-      char extra[128];
-      snprintf(extra, sizeof(extra), "\"Normalize\": %s, \"Translate\": %.17g, ",
-               f.Bool("normalize", false) ? "true" : "false", tr);
-      js.insert(at + key.size(), std::string(extra) + "\"SerializationPrefix\": \"" +
-                                     esc + "\", \"Limit\": " + std::to_string(limit) + ", ");
-    }
     std::ofstream ofs(json);
-    ofs << js << "\n";
+    ofs << WithInputOptions(stats.data(), f, limit, tr, "") << "\n";
   }
   hd_context_destroy(ctx);
   return res.check_pass == 0 ? 2 : 0;

@@ -104,3 +104,32 @@ def test_repeat_bounds_and_feature_vector(ctx):
     v = hd.features.feature_vector(res.stats, repeat=1)
     assert len(v) == hd.features.N_FEATURES and all(np.isfinite(v))
     assert v[1] == 20000 and v[24] == 15000  # NumPoints of each file
+
+
+def test_cli_vary_params_sweep(tmp_path):
+    from hausdorffdistance_amd import tuning
+
+    log = tmp_path / "train" / "n_points_cell" / "pair"
+    log.parent.mkdir(parents=True)
+    args = [str(EXE), "-input1", "uniform_100k_2d", "-input_type", "synthetic", "-n_dims", "2",
+            "-limit", "30000", "-repeat", "2", "-vary_params", "-n_points_cell_list", "4,8",
+            "-max_hit_list", "64,256", "-json", str(log)]
+    print("sweep", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    files = sorted(log.parent.glob("pair_sample_rate_*.json"))
+    assert [p.name for p in files] == sorted(
+        f"pair_sample_rate_0.010000_eb_only_threshold_0_max_hit_{h}_n_points_cell_{c}.json"
+        for h in (64, 256) for c in (4, 8))
+    runs = [json.loads(p.read_text()) for p in files]
+    hds = {run["HDResult"] for run in runs}
+    assert len(hds) == 1  # every combination gives the same distance
+    for run in runs:
+        assert run["Input"]["Files"][0]["Path"] == "uniform_100k_2d"
+        assert len(run["Running"]["Repeats"]) == 2 and "DateTime" in run
+    # A second sweep skips what exists.
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.count("Skip ") == 4
+    rows = tuning.training_rows(tuning.load_runs(str(tmp_path / "train"), "n_points_cell"))
+    assert len(rows) == 1
+    assert rows[0]["AvgTime"] == min(run["Running"]["AvgTime"] for run in runs)

@@ -184,3 +184,43 @@ def test_feature_vector_by_hand():
                                            {"value": 4, "percentile": 1.0, "count": 4}]
     got = feature_vector(stats)
     assert got[46:52] == [4.0, -1, -1, -1, -1, -1]
+
+
+def sweep_run(path_a, path_b, translate, avg, npc, max_hit):
+    rows = [{"value": v, "percentile": p, "count": c} for v, p, c in SEVEN]
+    f = {"Density": 1.0, "NumPoints": 7, "MBR": [{"Lower": 0, "Upper": 1}] * 2,
+         "Grid": {"GiniIndex": 0.1, "Histogram": rows, "GridSize": [2, 2],
+                  "NonEmptyCells": 4, "TotalCells": 4}}
+    return {"Input": {"NumDims": 2, "Translate": translate,
+                      "Files": [dict(f, Path=path_a), dict(f, Path=path_b)],
+                      "Grid": {"Histogram": rows, "GridSize": [2, 2]}},
+            "Running": {"AvgTime": avg, "SampleRate": 0.01, "EBOnlyThreshold": 0,
+                        "MaxHit": max_hit, "NumPointsPerCell": npc,
+                        "Repeats": [{"HDLowerBound": 0.5, "HDUpperBound": 1.0}]}}
+
+
+def test_tuning_export_picks_the_fastest_per_pair(tmp_path):
+    from hausdorffdistance_amd import tuning
+
+    d = tmp_path / "train" / "n_points_cell"
+    d.mkdir(parents=True)
+    runs = [sweep_run("a", "b", 0, 5.0, 4, 256), sweep_run("a", "b", 0, 3.0, 8, 256),
+            sweep_run("a", "b", 0, 4.0, 16, 256), sweep_run("a", "b", 0.1, 2.0, 16, 64),
+            sweep_run("a", "c", 0, 7.0, 2, 256)]
+    for i, r in enumerate(runs):
+        (d / f"x_{i}.json").write_text(json.dumps(r))
+    (d / "empty.json").write_text("")
+    (tmp_path / "train" / "other").mkdir()
+    (tmp_path / "train" / "other" / "y.json").write_text(json.dumps(runs[0]))
+    loaded = tuning.load_runs(str(tmp_path / "train"), "n_points_cell")
+    assert len(loaded) == 5
+    rows = {r["RunID"]: r for r in tuning.training_rows(loaded)}
+    assert set(rows) == {"a-b-0", "a-b-0.1", "a-c-0"}
+    assert rows["a-b-0"]["NumPointsPerCell"] == 8 and rows["a-b-0"]["AvgTime"] == 3.0
+    assert rows["a-b-0.1"]["MaxHit"] == 64
+    out = tmp_path / "t.csv"
+    tuning.export_csv(list(rows.values()), str(out))
+    lines = out.read_text().splitlines()
+    assert len(lines) == 4 and lines[0].split(",")[:6] == [
+        "RunID", "AvgTime", "SampleRate", "EBOnlyThreshold", "MaxHit", "NumPointsPerCell"]
+    assert len(lines[0].split(",")) == 6 + 63
