@@ -14,6 +14,7 @@ from .api import (  # noqa: F401
     HausdorffDistanceGrid,
     HausdorffDistanceHybrid,
     HausdorffDistanceLbvh,
+    HausdorffDistanceNearestNeighborSearch,
     Model,
     Result,
     RunConfig,

@@ -28,6 +28,7 @@ class Variant(IntEnum):
     GRID = _lib.HD_VARIANT_GRID
     LBVH = _lib.HD_VARIANT_LBVH
     HYBRID = _lib.HD_VARIANT_HYBRID
+    NN = _lib.HD_VARIANT_NN
 
     @staticmethod
     def from_string(s: str) -> "Variant":
@@ -35,7 +36,7 @@ class Variant(IntEnum):
         (a software LBVH here)."""
         key = s.lower()
         table = {"eb": Variant.EB, "rt": Variant.LBVH, "lbvh": Variant.LBVH,
-                 "hybrid": Variant.HYBRID, "grid": Variant.GRID}
+                 "hybrid": Variant.HYBRID, "grid": Variant.GRID, "nn": Variant.NN}
         if key not in table:
             raise HDError(_lib.HD_EINVAL, f"Unknown variant: {s}")
         return table[key]
@@ -321,6 +322,13 @@ class HausdorffDistanceLbvh(HausdorffDistance):
 class HausdorffDistanceHybrid(HausdorffDistance):
     variant = Variant.HYBRID
     algorithm = "Hybrid"
+
+
+class HausdorffDistanceNearestNeighborSearch(HausdorffDistance):
+    """The baseline of hausdorff_distance_nearest_neighbor_search.h: every A
+    point's exact NN, no early break."""
+    variant = Variant.NN
+    algorithm = "Nearest Neighbor Search"
 
 
 def batch_directed(ctx: Context, points_a, offs_a, points_b, offs_b) -> list:

@@ -32,6 +32,8 @@ const char* AlgorithmName(int variant) {
       return "LBVH";
     case HD_VARIANT_HYBRID:
       return "Hybrid";
+    case HD_VARIANT_NN:
+      return "Nearest Neighbor Search";
   }
   return "Unknown";
 }
@@ -81,7 +83,7 @@ int AcquireContext(hd_context* given, ScopedContext* sc) {
   return HD_OK;
 }
 
-bool ValidVariant(int v) { return v >= HD_VARIANT_EB && v <= HD_VARIANT_HYBRID; }
+bool ValidVariant(int v) { return v >= HD_VARIANT_EB && v <= HD_VARIANT_NN; }
 
 int ValidateSizes(uint64_t n_a, uint64_t n_b) {
   if (n_a == 0 || n_b == 0)
@@ -105,6 +107,8 @@ int Dispatch(hd_context* ctx, int variant, const DirectedArgs& args,
       return RunLbvh(ctx, args, out);
     case HD_VARIANT_HYBRID:
       return RunHybrid(ctx, args, out);
+    case HD_VARIANT_NN:
+      return RunNearestNeighborSearch(ctx, args, out);
   }
   return Fail(HD_EINVAL, "unknown variant %d", variant);
 }

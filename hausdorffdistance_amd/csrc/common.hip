@@ -236,7 +236,8 @@ int HostMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
                 h[2 * n_dims]);
   for (int d = 0; d < 2 * n_dims; ++d) mbr[d] = FromOrderedBits(h[d]);
   for (int d = 0; d < n_dims; ++d)
-    if (mbr[n_dims + d] - mbr[d] == 0.0f) mbr[n_dims + d] = mbr[d] + 1e-5f;
+    if (mbr[n_dims + d] - mbr[d] == 0.0f)
+      mbr[n_dims + d] = (float)((double)mbr[d] + 1e-5);  // mbr.h:288-296
   return HD_OK;
 }
 

@@ -175,6 +175,10 @@ int RunLbvh(hd_context* ctx, const DirectedArgs& args,
             hd_directed_result* out);
 int RunHybrid(hd_context* ctx, const DirectedArgs& args,
               hd_directed_result* out);
+// The nearest-neighbour-search baseline: exact LBVH NN of every A point
+// (no early break), then the maximum (nn.hip).
+int RunNearestNeighborSearch(hd_context* ctx, const DirectedArgs& args,
+                             hd_directed_result* out);
 // Grid query with early break whose queries that visit more than max_cells
 // cells are finished by the early-break scan (the reference's hybrid hands
 // such points from the RT query to its EB kernel, hybrid.h:291-295,377-400).

@@ -84,6 +84,54 @@ int NearestNeighbors(hd_context* ctx, int index, int n_dims, const float* q,
   return Fail(HD_EUNSUPPORTED, "index %d not built yet", index);
 }
 
+namespace {
+
+// Packed maximum (d2 bits << 32 | ~index) over the per-point NN distances:
+// the largest d2, and among equal d2 the smallest index. One atomic per wave.
+__global__ __launch_bounds__(256) void MaxOfMinKernel(
+    const float* __restrict__ d2, uint64_t n, unsigned long long* key) {
+  unsigned long long best = 0ull;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const float v = d2[i];
+    if (v != kFltMax) best = max(best, PackKey(v, (uint32_t)i));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(best, off, 64);
+    best = max(best, o);
+  }
+  if ((threadIdx.x & 63u) == 0 && best != 0ull) atomicMax(key, best);
+}
+
+}  // namespace
+
+// Reference hausdorff_distance_nearest_neighbor_search.h:107-153 (GPU):
+// a kd-tree over B, the closest point of every A point, the maximum. Here
+// the LBVH's exact NN query, then a packed-key maximum; the result bits and
+// the arg-max are those of every other variant.
+int RunNearestNeighborSearch(hd_context* ctx, const DirectedArgs& args,
+                             hd_directed_result* out) {
+  float* d2 = nullptr;
+  uint32_t* idx = nullptr;
+  unsigned long long* key = nullptr;
+  HD_TRY(ContextArray(ctx, "nns_d2", args.n_a, &d2));
+  HD_TRY(ContextArray(ctx, "nns_idx", args.n_a, &idx));
+  HD_TRY(ContextArray(ctx, "nns_key", 2, &key));
+  HD_TRY(FillU64(ctx, key, 0ull, 1));
+  HD_TRY(FillU64(ctx, key + 1, (unsigned long long)args.n_a, 1));
+  HD_TRY(LbvhNearestNeighbors(ctx, args.n_dims, args.a, args.n_a, args.b,
+                              args.n_b, d2, idx));
+  const int grid = (int)std::min<uint64_t>((args.n_a + 255) / 256,
+                                           (uint64_t)ctx->cu_count * 8);
+  MaxOfMinKernel<<<grid, 256, 0, ctx->stream>>>(d2, args.n_a, key);
+  HD_HIP_TRY(hipGetLastError());
+  HD_TRY(FinishDirected(ctx, args.n_dims, args.a, args.b, args.n_b, key,
+                        key + 1, 1, out));
+  out->variant_used = HD_VARIANT_NN;
+  return HD_OK;
+}
+
 }  // namespace hd
 
 namespace hd {

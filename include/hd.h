@@ -43,12 +43,16 @@ extern "C" {
 
 /* Variants. Reference enum Variant (src/run_config.h:7-16) and the CLI
  * strings of src/main.cpp:76-94: "eb" -> EB, "rt" -> LBVH (the OptiX BVH
- * query becomes a software LBVH), "hybrid" -> HYBRID. GRID is the uniform
- * grid index queried on its own. */
+ * query becomes a software LBVH), "hybrid" -> HYBRID, "nn" -> NN (the
+ * nearest-neighbour-search baseline, hausdorff_distance_nearest_neighbor_
+ * search.h:107-153: every A point's exact NN in B, no early break, then the
+ * maximum; the kd-tree becomes the LBVH). GRID is the uniform grid index
+ * queried on its own. */
 #define HD_VARIANT_EB 0
 #define HD_VARIANT_GRID 1
 #define HD_VARIANT_LBVH 2
 #define HD_VARIANT_HYBRID 3
+#define HD_VARIANT_NN 4
 
 /* Execution (src/run_config.h:18). This library serves GPU execution only;
  * HD_EXEC_CPU returns HD_EUNSUPPORTED. */

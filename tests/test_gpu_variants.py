@@ -1,5 +1,6 @@
-"""GPU parity of the index-based variants (grid, LBVH, hybrid) and of the
-exact nearest-neighbour entry point, through the C ABI, against the CPU
+"""GPU parity of the index-based variants (grid, LBVH, hybrid, the
+nearest-neighbour-search baseline) and of the exact nearest-neighbour entry
+point, through the C ABI, against the CPU
 oracle: cmax2 bits, arg-max and NN index must be identical."""
 from __future__ import annotations
 
@@ -15,7 +16,7 @@ from oracle import oracle
 pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
-VARIANTS = [hd.Variant.GRID, hd.Variant.LBVH, hd.Variant.HYBRID]
+VARIANTS = [hd.Variant.GRID, hd.Variant.LBVH, hd.Variant.HYBRID, hd.Variant.NN]
 
 
 def run(a, b, variant, symmetric=True, ctx=None, **kw):
