@@ -12,6 +12,9 @@ early-break kernel with the most device time: the matrix-core filter of the
 budgeted rounds, or the exact scan), its duration measured with HIP events on
 the library's stream; the CPU baseline is the CPU oracle (a C restatement of
 the reference's CPU early break) on a bounded sample of the same workload.
+The other variants (grid, LBVH, hybrid, nearest-neighbour search) and, under
+config.next_rows, the input-statistics pipeline and the loaders' parse
+rates are timed once each beside it; none of them enters `value`.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
@@ -87,6 +90,61 @@ def time_steps(hd, ctx, variant, da, db, steps, warmup, learned, model=None):
     stages = stage_times(lib, ctx, reset=True)
     lib.hd_context_set_profiling(ctx.handle, 0)
     return (t1 - t0) * 1e3 / steps, results, stages
+
+
+def next_rows(hd, ctx, da, db, n_a: int, n_b: int) -> dict:
+    """The widened rows around the hot path, each timed once on the same
+    workload: the input statistics of the stats JSON (three stats grids:
+    A, B and both, each one counting pass over its points) and the
+    loaders' parse rates (host code, on files written here)."""
+    import tempfile
+
+    import torch
+
+    lib = hd.load()
+    out = {}
+    cfg = hd.RunConfig(variant=hd.Variant.HYBRID, n_dims=2, repeat=1, stats_n_points_cell=10)
+    hd.run_hausdorff_distance(cfg, da, db, ctx=ctx)  # warm-up
+    torch.cuda.synchronize()
+    lib.hd_context_set_profiling(ctx.handle, 1)
+    stage_times(lib, ctx, reset=True)
+    res = hd.run_hausdorff_distance(cfg, da, db, ctx=ctx)
+    st = stage_times(lib, ctx, reset=True)
+    lib.hd_context_set_profiling(ctx.handle, 0)
+    s = st.get("input_stats")
+    if s:
+        pts = 2 * (n_a + n_b)
+        grid = res.stats["Input"]["Grid"]
+        out["input_stats"] = {
+            "ms": s["ms"], "points_counted": pts,
+            "points_per_s": pts / (s["ms"] / 1e3),
+            # point reads (8 B) and the count tables' clear, count and
+            # histogram passes (4 B per cell each)
+            "gb_per_s": (pts * 8 + 3 * 4 * 2 * grid["TotalCells"]) / (s["ms"] / 1e3) / 1e9,
+            "union_grid": {k: grid[k] for k in ("GridSize", "NonEmptyCells", "MaxPoints")}}
+    rng = np.random.default_rng(1)
+    with tempfile.TemporaryDirectory() as tmp:
+        pts = rng.random((1_000_000, 2))
+        wkt = os.path.join(tmp, "a.wkt")
+        with open(wkt, "w") as f:
+            for i in range(0, len(pts), 100):
+                f.write("LINESTRING (" + ", ".join(f"{x:.9f} {y:.9f}" for x, y in pts[i:i + 100])
+                        + ")\n")
+        ply = os.path.join(tmp, "a.ply")
+        v = rng.random((2_000_000, 3)).astype("<f4")
+        with open(ply, "wb") as f:
+            f.write((f"ply\nformat binary_little_endian 1.0\nelement vertex {len(v)}\n"
+                     "property float x\nproperty float y\nproperty float z\nend_header\n").encode())
+            f.write(v.tobytes())
+        for name, path, kind, nd, n in (("wkt", wkt, "wkt", 2, len(pts)),
+                                        ("ply_binary", ply, "ply", 3, len(v))):
+            t0 = time.perf_counter()
+            p = hd.load_points(path, kind, nd)
+            dt = time.perf_counter() - t0
+            assert p.shape == (n, nd)
+            out[f"load_{name}"] = {"points": n, "s": dt, "points_per_s": n / dt,
+                                   "mb_per_s": os.path.getsize(path) / dt / 1e6}
+    return out
 
 
 def eb_roofline(stages: dict, steps: int) -> dict:
@@ -275,7 +333,9 @@ def main() -> None:
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--variants", default="grid,lbvh,hybrid",
+    ap.add_argument("--no-next", action="store_true",
+                    help="skip the input-statistics and loader timings")
+    ap.add_argument("--variants", default="grid,lbvh,hybrid,nn",
                     help="other variants timed on the same workload")
     ap.add_argument("--no-symmetric", action="store_true",
                     help="skip the symmetric hybrid measurement")
@@ -385,6 +445,14 @@ def main() -> None:
     except hd.HDError as e:
         log(f"symmetric hybrid unavailable: {e}")
 
+    extra = None
+    if not args.no_next:
+        try:
+            extra = next_rows(hd, ctx, da, db, a.shape[0], b.shape[0])
+            log(f"next rows: {json.dumps(extra)}")
+        except hd.HDError as e:
+            log(f"next rows unavailable: {e}")
+
     use_learned = ms_learned is not None
     ms = ms_learned if use_learned else ms_plain
     stages = st_learned if use_learned else st_plain
@@ -432,6 +500,7 @@ def main() -> None:
             "stages_ms": {k: v["ms"] / max(1, v["calls"]) for k, v in stages.items()},
             "other_variants": other,
             "symmetric_hybrid": sym,
+            "next_rows": extra,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -577,9 +577,12 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
     const float density = (float)(n_a + n_b) / MbrVolume(nd, lo, hi);
     const StatsInput sets[2] = {{d_a, n_a}, {d_b, n_b}};
     const uint32_t spc = (uint32_t)std::max(cfg->stats_n_points_cell, 0);
+    const int stage = StageBegin(c, "input_stats");
     HD_TRY(InputStatsJson(c, nd, &sets[0], 1, mbr_a, spc, nullptr, &files[0]));
     HD_TRY(InputStatsJson(c, nd, &sets[1], 1, mbr_b, spc, nullptr, &files[1]));
     HD_TRY(InputStatsJson(c, nd, sets, 2, mbr_ab, spc, &density, &both));
+    StageEnd(c, stage);
+    StageCount(c, "input_stats", 0, 2 * (n_a + n_b));
     both.pop_back();  // the union's object, extended below
     std::ostringstream js;
     js << "{\"GPU\": {\"Device\": " << c->device << ", \"name\": \""

@@ -57,9 +57,13 @@ __global__ void StatsCellKernel(StatsGridDev g, const float* __restrict__ pts,
   }
 }
 
-__global__ void CellCountKernel(const unsigned int* __restrict__ counts,
-                                uint64_t n_cells, unsigned int* nonempty,
-                                unsigned int* max_count) {
+// Non-empty cells and the largest count. Grid-stride over the table with a
+// few blocks per CU; wave then block sums, one pair of atomics per block
+// (an atomic per wave on two addresses serialises into the kernel's cost).
+__global__ __launch_bounds__(256) void CellCountKernel(
+    const unsigned int* __restrict__ counts, uint64_t n_cells,
+    unsigned int* nonempty, unsigned int* max_count) {
+  __shared__ unsigned int s_ne[4], s_mx[4];
   unsigned int ne = 0, mx = 0;
   for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
        c < n_cells; c += (uint64_t)gridDim.x * blockDim.x) {
@@ -67,25 +71,50 @@ __global__ void CellCountKernel(const unsigned int* __restrict__ counts,
     ne += k > 0u;
     mx = max(mx, k);
   }
-  // Wave sums, one atomic per wave.
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     ne += __shfl_xor(ne, off, 64);
     mx = max(mx, (unsigned int)__shfl_xor(mx, off, 64));
   }
+  const unsigned int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63u) == 0) {
+    s_ne[wave] = ne;
+    s_mx[wave] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ne = s_ne[0] + s_ne[1] + s_ne[2] + s_ne[3];
+    mx = max(max(s_mx[0], s_mx[1]), max(s_mx[2], s_mx[3]));
     if (ne) atomicAdd(nonempty, ne);
     if (mx) atomicMax(max_count, mx);
   }
 }
 
-__global__ void CellHistogramKernel(const unsigned int* __restrict__ counts,
-                                    uint64_t n_cells, unsigned int* hist) {
+// Cells per count. Most cells hold few points, so a handful of bins take
+// nearly every update: counts below kHistBins go to a per-block LDS
+// histogram that is added to the global one at the end; larger counts
+// (rare) go to the global one directly. hist has max_count + 1 entries.
+constexpr unsigned int kHistBins = 4096;
+
+__global__ __launch_bounds__(256) void CellHistogramKernel(
+    const unsigned int* __restrict__ counts, uint64_t n_cells,
+    unsigned int* hist, unsigned int max_count) {
+  __shared__ unsigned int bins[kHistBins];
+  for (unsigned int i = threadIdx.x; i < kHistBins; i += blockDim.x) bins[i] = 0u;
+  __syncthreads();
   for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
        c < n_cells; c += (uint64_t)gridDim.x * blockDim.x) {
     const unsigned int k = counts[c];
-    if (k > 0u) atomicAdd(&hist[k], 1u);
+    if (k == 0u) continue;
+    if (k < kHistBins)
+      atomicAdd(&bins[k], 1u);
+    else
+      atomicAdd(&hist[k], 1u);
   }
+  __syncthreads();
+  const unsigned int lim = min(max_count + 1u, kHistBins);
+  for (unsigned int i = threadIdx.x; i < lim; i += blockDim.x)
+    if (bins[i]) atomicAdd(&hist[i], bins[i]);
 }
 
 std::string Num(double v) {
@@ -298,7 +327,8 @@ int InputStatsJson(hd_context* ctx, int nd, const StatsInput* sets, int n_sets,
       StatsCellKernel<3><<<pb, 256, 0, ctx->stream>>>(g, sets[s].pts, sets[s].n, counts);
     HD_HIP_TRY(hipGetLastError());
   }
-  const int cb = (int)std::min<uint64_t>((n_cells + 255) / 256, 8192);
+  const int cb = (int)std::min<uint64_t>((n_cells + 255) / 256,
+                                         (uint64_t)std::max(1, ctx->cu_count) * 4);
   CellCountKernel<<<cb, 256, 0, ctx->stream>>>(counts, n_cells, counters,
                                                counters + 1);
   HD_HIP_TRY(hipGetLastError());
@@ -312,7 +342,8 @@ int InputStatsJson(hd_context* ctx, int nd, const StatsInput* sets, int n_sets,
     unsigned int* d_hist = nullptr;
     HD_TRY(ContextArray(ctx, "stats_hist", max_count + 1, &d_hist));
     HD_TRY(FillU32(ctx, d_hist, 0u, max_count + 1));
-    CellHistogramKernel<<<cb, 256, 0, ctx->stream>>>(counts, n_cells, d_hist);
+    CellHistogramKernel<<<cb, 256, 0, ctx->stream>>>(counts, n_cells, d_hist,
+                                                     max_count);
     HD_HIP_TRY(hipGetLastError());
     std::vector<unsigned int> hh(max_count + 1);
     HD_HIP_TRY(hipMemcpyAsync(hh.data(), d_hist, sizeof(unsigned int) * hh.size(),
