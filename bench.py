@@ -13,8 +13,9 @@ budgeted rounds, or the exact scan), its duration measured with HIP events on
 the library's stream; the CPU baseline is the CPU oracle (a C restatement of
 the reference's CPU early break) on a bounded sample of the same workload.
 The other variants (grid, LBVH, hybrid, nearest-neighbour search) and, under
-config.next_rows, the input-statistics pipeline and the loaders' parse
-rates are timed once each beside it; none of them enters `value`.
+config.next_rows, the input-statistics pipeline, the RT-HDIST baseline (on
+1M x 1M uniform 2D) and the loaders' parse rates are timed beside it; none
+of them enters `value`.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
@@ -122,6 +123,18 @@ def next_rows(hd, ctx, da, db, n_a: int, n_b: int) -> dict:
             # histogram passes (4 B per cell each)
             "gb_per_s": (pts * 8 + 3 * 4 * 2 * grid["TotalCells"]) / (s["ms"] / 1e3) / 1e9,
             "union_grid": {k: grid[k] for k in ("GridSize", "NonEmptyCells", "MaxPoints")}}
+    # RT-HDIST on 1M x 1M uniform 2D (on the clustered 10M pair its scan of
+    # every cell within three diagonals is millions of evaluations per point).
+    ua = torch.from_numpy(hd.gen.uniform(1, 1_000_000, 2)).cuda()
+    ub = torch.from_numpy(hd.gen.uniform(2, 1_000_000, 2)).cuda()
+    ms_r, res_r, st_r = time_steps(hd, ctx, hd.Variant.RT_HDIST, ua, ub, 3, 1, learned=False)
+    ms_e, res_e, _ = time_steps(hd, ctx, hd.Variant.HYBRID, ua, ub, 3, 1, learned=False)
+    rr, re_ = res_r[-1], res_e[-1]
+    out["rt_hdist_uniform_1m_2d"] = {
+        "ms_per_step": ms_r, "a_points_per_s": 1e6 / (ms_r / 1e3), "evals": rr.evals,
+        "exact": (rr.cmax2_bits, rr.argmax) == (re_.cmax2_bits, re_.argmax),
+        "hybrid_ms_per_step": ms_e,
+        "stages_ms": {k: x["ms"] / max(1, x["calls"]) for k, x in st_r.items()}}
     rng = np.random.default_rng(1)
     with tempfile.TemporaryDirectory() as tmp:
         pts = rng.random((1_000_000, 2))

@@ -15,6 +15,7 @@ from .api import (  # noqa: F401
     HausdorffDistanceHybrid,
     HausdorffDistanceLbvh,
     HausdorffDistanceNearestNeighborSearch,
+    HausdorffDistanceRTHDist,
     Model,
     Result,
     RunConfig,

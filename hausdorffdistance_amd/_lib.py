@@ -24,6 +24,7 @@ HD_VARIANT_GRID = 1
 HD_VARIANT_LBVH = 2
 HD_VARIANT_HYBRID = 3
 HD_VARIANT_NN = 4
+HD_VARIANT_RT_HDIST = 5
 
 HD_EXEC_CPU = 0
 HD_EXEC_GPU = 1

@@ -29,6 +29,7 @@ class Variant(IntEnum):
     LBVH = _lib.HD_VARIANT_LBVH
     HYBRID = _lib.HD_VARIANT_HYBRID
     NN = _lib.HD_VARIANT_NN
+    RT_HDIST = _lib.HD_VARIANT_RT_HDIST
 
     @staticmethod
     def from_string(s: str) -> "Variant":
@@ -36,7 +37,8 @@ class Variant(IntEnum):
         (a software LBVH here)."""
         key = s.lower()
         table = {"eb": Variant.EB, "rt": Variant.LBVH, "lbvh": Variant.LBVH,
-                 "hybrid": Variant.HYBRID, "grid": Variant.GRID, "nn": Variant.NN}
+                 "hybrid": Variant.HYBRID, "grid": Variant.GRID, "nn": Variant.NN,
+                 "rt-hdist": Variant.RT_HDIST}
         if key not in table:
             raise HDError(_lib.HD_EINVAL, f"Unknown variant: {s}")
         return table[key]
@@ -322,6 +324,13 @@ class HausdorffDistanceLbvh(HausdorffDistance):
 class HausdorffDistanceHybrid(HausdorffDistance):
     variant = Variant.HYBRID
     algorithm = "Hybrid"
+
+
+class HausdorffDistanceRTHDist(HausdorffDistance):
+    """The RT-HDIST baseline (hausdorff_distance_rt_hdist.h) on a quantized
+    grid; its result is at least the exact HD."""
+    variant = Variant.RT_HDIST
+    algorithm = "RT-HDIST"
 
 
 class HausdorffDistanceNearestNeighborSearch(HausdorffDistance):

@@ -34,6 +34,8 @@ const char* AlgorithmName(int variant) {
       return "Hybrid";
     case HD_VARIANT_NN:
       return "Nearest Neighbor Search";
+    case HD_VARIANT_RT_HDIST:
+      return "RT-HDIST";
   }
   return "Unknown";
 }
@@ -83,7 +85,7 @@ int AcquireContext(hd_context* given, ScopedContext* sc) {
   return HD_OK;
 }
 
-bool ValidVariant(int v) { return v >= HD_VARIANT_EB && v <= HD_VARIANT_NN; }
+bool ValidVariant(int v) { return v >= HD_VARIANT_EB && v <= HD_VARIANT_RT_HDIST; }
 
 int ValidateSizes(uint64_t n_a, uint64_t n_b) {
   if (n_a == 0 || n_b == 0)
@@ -109,6 +111,8 @@ int Dispatch(hd_context* ctx, int variant, const DirectedArgs& args,
       return RunHybrid(ctx, args, out);
     case HD_VARIANT_NN:
       return RunNearestNeighborSearch(ctx, args, out);
+    case HD_VARIANT_RT_HDIST:
+      return RunRtHdist(ctx, args, out);
   }
   return Fail(HD_EINVAL, "unknown variant %d", variant);
 }

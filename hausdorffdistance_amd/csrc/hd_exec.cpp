@@ -270,7 +270,7 @@ int main(int argc, char** argv) {
   if (argc == 1 || !Parse(argc, argv, &f)) {
     fprintf(stderr,
             "Usage: hd_exec -input1 A -input2 B -input_type wkt|off|ply|bin|synthetic "
-            "-n_dims 2|3 [-variant eb|grid|rt|lbvh|hybrid|nn] [-execution gpu] "
+            "-n_dims 2|3 [-variant eb|grid|rt|lbvh|hybrid|nn|rt-hdist] [-execution gpu] "
             "[-repeat 5] [-seed 0] [-check] [-symmetric] [-learned_order] "
             "[-n_points_cell 8] [-max_hit 256] [-auto_tune] [-limit N] "
             "[-stats_n_points_cell 10] [-vary_params -sample_rate_list r,.. "
@@ -293,6 +293,7 @@ int main(int argc, char** argv) {
   else if (variant == "rt" || variant == "lbvh") cfg.variant = HD_VARIANT_LBVH;
   else if (variant == "hybrid") cfg.variant = HD_VARIANT_HYBRID;
   else if (variant == "nn") cfg.variant = HD_VARIANT_NN;
+  else if (variant == "rt-hdist") cfg.variant = HD_VARIANT_RT_HDIST;
   else {
     fprintf(stderr, "Unknown variant: %s\n", variant.c_str());
     return 1;

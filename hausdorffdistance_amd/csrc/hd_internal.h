@@ -179,6 +179,8 @@ int RunHybrid(hd_context* ctx, const DirectedArgs& args,
 // (no early break), then the maximum (nn.hip).
 int RunNearestNeighborSearch(hd_context* ctx, const DirectedArgs& args,
                              hd_directed_result* out);
+// The RT-HDIST baseline on a quantized grid (rthdist.hip).
+int RunRtHdist(hd_context* ctx, const DirectedArgs& args, hd_directed_result* out);
 // Grid query with early break whose queries that visit more than max_cells
 // cells are finished by the early-break scan (the reference's hybrid hands
 // such points from the RT query to its EB kernel, hybrid.h:291-295,377-400).

@@ -40,11 +40,6 @@ int GatherSoA(hd_context* ctx, int n_dims, const float* aos, uint32_t n,
               uint32_t n_pad, bool shuffle, const Feistel& perm,
               float* x, float* y, float* z);
 
-// Tight MBR of an AoS point set into a device array lower[d], upper[d]
-// (single-pass reduction: wave, then block, then one atomic per block).
-int DeviceMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
-              float* d_mbr /* 2*n_dims floats, device */);
-
 // Reads the MBR back (synchronous) and applies the reference's
 // ToNonemptyMBR widening (src/geoms/mbr.h:288-296).
 int HostMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,

@@ -53,6 +53,12 @@ extern "C" {
 #define HD_VARIANT_LBVH 2
 #define HD_VARIANT_HYBRID 3
 #define HD_VARIANT_NN 4
+/* "rt-hdist": the quantized-grid baseline (hausdorff_distance_rt_hdist.h,
+ * shaders_nn_quantized_grid_{2,3}d.cu) without ray-tracing hardware. Its
+ * result is at least the exact HD and equals it on ordinary inputs (each A
+ * point's minimum is taken over the B points of the quantized cells within
+ * three cell diagonals beyond its first hit). */
+#define HD_VARIANT_RT_HDIST 5
 
 /* Execution (src/run_config.h:18). This library serves GPU execution only;
  * HD_EXEC_CPU returns HD_EUNSUPPORTED. */

@@ -130,3 +130,37 @@ def test_hybrid_escalates_far_queries(ctx):
     o = oracle.eb_directed(b, a)
     r = run(b, a, hd.Variant.HYBRID, ctx=ctx, symmetric=False, max_hit=4)
     assert triple(r.ab) == triple(o)
+
+
+RTH_CASES = {
+    "uniform_2d": lambda: (hd.gen.uniform(101, 2000, 2), hd.gen.uniform(102, 1500, 2)),
+    "uniform_3d": lambda: (hd.gen.uniform(103, 900, 3), hd.gen.uniform(104, 700, 3) * np.float32(0.5)),
+    # A beside B: the first launches miss and the radius grows many times.
+    "separated_2d": lambda: (hd.gen.uniform(105, 300, 2) + np.float32(2.5),
+                             hd.gen.uniform(106, 400, 2)),
+    # A around B on every side, some points left of and below it (negative
+    # offsets truncate toward zero).
+    "around_2d": lambda: (hd.gen.uniform(107, 600, 2) * np.float32(3) - np.float32(1),
+                          hd.gen.uniform(108, 500, 2)),
+    "flat_b_2d": lambda: (hd.gen.uniform(109, 500, 2),
+                          np.c_[hd.gen.uniform(110, 400, 2)[:, :1], np.full(400, 0.5, np.float32)]),
+    "two_b": lambda: (hd.gen.uniform(111, 50, 2), hd.gen.uniform(112, 2, 2)),
+    "one_a": lambda: (hd.gen.uniform(113, 1, 3), hd.gen.uniform(114, 300, 3)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(RTH_CASES))
+def test_rt_hdist_matches_its_restatement(ctx, case):
+    from oracle import rthdist
+
+    a, b = (np.ascontiguousarray(x, np.float32) for x in RTH_CASES[case]())
+    print(f"rt-hdist {case} {a.shape} {b.shape}", flush=True)
+    r = run(a, b, hd.Variant.RT_HDIST, symmetric=False, ctx=ctx)
+    bits, arg = rthdist.rt_hdist(a, b)
+    assert (r.ab.cmax2_bits, r.ab.argmax) == (bits, arg)
+    # On these inputs every A point's nearest B point lies within the scan
+    # radius, so the baseline returns the exact distance (and the NN of the
+    # arg-max is the exact one).
+    exact = oracle.brute_directed(a, b)
+    assert (bits, arg) == (exact.cmax2_bits, exact.argmax)
+    assert r.ab.nn == exact.nn
