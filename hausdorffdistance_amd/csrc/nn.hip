@@ -87,9 +87,11 @@ int NearestNeighbors(hd_context* ctx, int index, int n_dims, const float* q,
 namespace {
 
 // Packed maximum (d2 bits << 32 | ~index) over the per-point NN distances:
-// the largest d2, and among equal d2 the smallest index. One atomic per wave.
+// the largest d2, and among equal d2 the smallest index. One atomic per
+// block (blockDim 256).
 __global__ __launch_bounds__(256) void MaxOfMinKernel(
     const float* __restrict__ d2, uint64_t n, unsigned long long* key) {
+  __shared__ unsigned long long part[4];
   unsigned long long best = 0ull;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
@@ -101,7 +103,12 @@ __global__ __launch_bounds__(256) void MaxOfMinKernel(
     const unsigned long long o = __shfl_xor(best, off, 64);
     best = max(best, o);
   }
-  if ((threadIdx.x & 63u) == 0 && best != 0ull) atomicMax(key, best);
+  if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    best = max(max(part[0], part[1]), max(part[2], part[3]));
+    if (best != 0ull) atomicMax(key, best);
+  }
 }
 
 }  // namespace
@@ -123,7 +130,7 @@ int RunNearestNeighborSearch(hd_context* ctx, const DirectedArgs& args,
   HD_TRY(LbvhNearestNeighbors(ctx, args.n_dims, args.a, args.n_a, args.b,
                               args.n_b, d2, idx));
   const int grid = (int)std::min<uint64_t>((args.n_a + 255) / 256,
-                                           (uint64_t)ctx->cu_count * 8);
+                                           (uint64_t)ctx->cu_count * 2);
   MaxOfMinKernel<<<grid, 256, 0, ctx->stream>>>(d2, args.n_a, key);
   HD_HIP_TRY(hipGetLastError());
   HD_TRY(FinishDirected(ctx, args.n_dims, args.a, args.b, args.n_b, key,

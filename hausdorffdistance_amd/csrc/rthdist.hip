@@ -134,8 +134,9 @@ __global__ void MaxF32Kernel(const float* __restrict__ v, uint32_t n,
   if ((threadIdx.x & 63u) == 0) atomicMax(out_bits, __float_as_uint(m));
 }
 
-// Scan of one A point: first k with dmin2 <= r_k^2, radius r_{k+3}, exact
-// minimum over the points of every non-empty cell within it.
+// Scan of one A point per wave: first k with dmin2 <= r_k^2, radius
+// r_{k+3}; the cells within it are walked in order (uniform across the
+// wave) and the lanes stride over each cell's points, then a wave minimum.
 template <int ND>
 __global__ __launch_bounds__(256) void RtHdistScanKernel(
     QGrid g, const float* __restrict__ a, uint32_t n_a,
@@ -145,9 +146,10 @@ __global__ __launch_bounds__(256) void RtHdistScanKernel(
     const float* __restrict__ bz, const uint32_t* __restrict__ start,
     const uint32_t* __restrict__ end, unsigned long long* key,
     unsigned long long* evals) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
   unsigned long long my_evals = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_a;
-       i += gridDim.x * blockDim.x) {
+  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n_a; i += n_waves) {
     const float dm = dmin2[i];
     uint32_t lo = 0, hi = n_radius - 4;  // radius[hi]^2 >= max dmin2
     while (lo < hi) {
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(256) void RtHdistScanKernel(
               (uint32_t)cx + (uint32_t)g.res[0] *
                                  ((uint32_t)cy + (ND == 3 ? (uint32_t)g.res[1] * (uint32_t)cz : 0u));
           const uint32_t s = start[cell], e = end[cell];
-          for (uint32_t j = s; j < e; ++j) {
+          for (uint32_t j = s + lane; j < e; j += 64u) {
             const float d2 = ND == 2 ? D2(p[0], p[1], bx[j], by[j])
                                      : D2(p[0], p[1], p[2], bx[j], by[j], bz[j]);
             best = fminf(best, d2);
@@ -192,15 +194,18 @@ __global__ __launch_bounds__(256) void RtHdistScanKernel(
         }
       }
     }
-    if (best != kFltMax) atomicMax(key, PackKey(best, i));
-  }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) my_evals += __shfl_xor(my_evals, off, 64);
-  if ((threadIdx.x & 63u) == 0 && my_evals) atomicAdd(evals, my_evals);
+    for (int off = 32; off > 0; off >>= 1) best = fminf(best, __shfl_xor(best, off, 64));
+    if (lane == 0 && best != kFltMax) atomicMax(key, PackKey(best, i));
+  }
+  // my_evals is the same in every lane of a wave: one lane counts it.
+  if (lane == 0 && my_evals) atomicAdd(evals, my_evals);
 }
 
 template <int ND>
-__global__ void SumKernel(const float* __restrict__ p, uint64_t n, double* sums) {
+__global__ __launch_bounds__(256) void SumKernel(const float* __restrict__ p, uint64_t n,
+                                                 double* sums) {
+  __shared__ double part[4][3];
   double s[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) s[d] = 0.0;
@@ -212,8 +217,12 @@ __global__ void SumKernel(const float* __restrict__ p, uint64_t n, double* sums)
   for (int d = 0; d < ND; ++d) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s[d] += __shfl_xor(s[d], off, 64);
-    if ((threadIdx.x & 63u) == 0) atomicAdd(&sums[d], s[d]);
+    if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6][d] = s[d];
   }
+  __syncthreads();
+  if (threadIdx.x < (unsigned)ND)
+    atomicAdd(&sums[threadIdx.x], part[0][threadIdx.x] + part[1][threadIdx.x] +
+                                      part[2][threadIdx.x] + part[3][threadIdx.x]);
 }
 
 }  // namespace
@@ -228,7 +237,8 @@ int RunRtHdist(hd_context* ctx, const DirectedArgs& args, hd_directed_result* ou
   double* sums = nullptr;
   HD_TRY(ContextArray(ctx, "rth_sums", 6, &sums));
   HD_TRY(FillU64(ctx, reinterpret_cast<unsigned long long*>(sums), 0ull, 6));
-  const int sb = (int)std::min<uint64_t>((std::max(n_a, n_b) + 255) / 256, 2048);
+  const int sb = (int)std::min<uint64_t>((std::max(n_a, n_b) + 255) / 256,
+                                         (uint64_t)ctx->cu_count * 2);
   if (nd == 2) {
     SumKernel<2><<<sb, 256, 0, ctx->stream>>>(args.a, n_a, sums);
     SumKernel<2><<<sb, 256, 0, ctx->stream>>>(args.b, n_b, sums + 3);
@@ -358,7 +368,7 @@ int RunRtHdist(hd_context* ctx, const DirectedArgs& args, hd_directed_result* ou
   unsigned long long* key = nullptr;
   HD_TRY(ContextArray(ctx, "rth_key", 2, &key));
   HD_TRY(FillU64(ctx, key, 0ull, 2));
-  const int scan_blocks = (int)std::min<uint64_t>((n_a + 255) / 256, (uint64_t)ctx->cu_count * 8);
+  const int scan_blocks = (int)std::min<uint64_t>((n_a + 3) / 4, (uint64_t)ctx->cu_count * 8);
   if (nd == 2)
     RtHdistScanKernel<2><<<scan_blocks, 256, 0, ctx->stream>>>(
         g, args.a, n_a, qa, dmin2, d_radius, (uint32_t)radius.size(), bx, by, bz, start, end,
