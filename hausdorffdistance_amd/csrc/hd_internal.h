@@ -175,7 +175,7 @@ int RunLbvh(hd_context* ctx, const DirectedArgs& args,
             hd_directed_result* out);
 int RunHybrid(hd_context* ctx, const DirectedArgs& args,
               hd_directed_result* out);
-// The nearest-neighbour-search baseline: exact LBVH NN of every A point
+// The nearest-neighbour-search baseline: exact grid NN of every A point
 // (no early break), then the maximum (nn.hip).
 int RunNearestNeighborSearch(hd_context* ctx, const DirectedArgs& args,
                              hd_directed_result* out);

@@ -46,7 +46,7 @@ extern "C" {
  * query becomes a software LBVH), "hybrid" -> HYBRID, "nn" -> NN (the
  * nearest-neighbour-search baseline, hausdorff_distance_nearest_neighbor_
  * search.h:107-153: every A point's exact NN in B, no early break, then the
- * maximum; the kd-tree becomes the LBVH). GRID is the uniform grid index
+ * maximum; the kd-tree becomes the uniform grid). GRID is the uniform grid index
  * queried on its own. */
 #define HD_VARIANT_EB 0
 #define HD_VARIANT_GRID 1
