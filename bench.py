@@ -192,7 +192,14 @@ def eb_roofline(stages: dict, steps: int) -> dict:
             "peak": MFMA_F32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / MFMA_F32_TFLOPS if achieved else None,
             "traffic": traffic, "algorithmic_flop": flop, "kernel_ms": t_kernel * 1e3,
-            "launches_per_step": calls / max(1, steps), "kernels": other,
+            "launches_per_step": calls / max(1, steps),
+            # the other two views of the same launches: the streaming model
+            # (8 B of B per evaluated pair, as if nothing were reused) and the
+            # measured HBM rate (calibrated counters over the launch time)
+            "streaming_model_frac": (8.0 * evals / launches / t_kernel / 1e9 / HBM_PEAK_GBS)
+            if t_kernel > 0 else None,
+            "hbm_measured_gbps": (traffic / t_kernel / 1e9) if traffic and t_kernel > 0 else None,
+            "kernels": other,
         }
     bytes_alg = 8.0 * (points + evals) / launches
     achieved = bytes_alg / t_kernel / 1e9 if t_kernel > 0 else None
