@@ -37,7 +37,9 @@ def test_cli_wkt_with_cache(tmp_path):
     pa = hd.load_points(fa, "wkt", 2)
     pb = hd.load_points(fb, "wkt", 2)
     o = oracle.eb_directed(pa, pb)
-    for variant in ("eb", "hybrid", "rt"):
+    # rt-hdist is exact on these uniform inputs (its scan radius covers
+    # every nearest point), like the other variants by construction.
+    for variant in ("eb", "hybrid", "rt", "grid", "nn", "rt-hdist"):
         got = run_cli("-input1", fa, "-input2", fb, "-input_type", "wkt", "-n_dims", "2",
                       "-variant", variant, "-repeat", "1", "-check",
                       "-serialize", str(tmp_path / "cache"))
