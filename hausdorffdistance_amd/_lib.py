@@ -145,6 +145,9 @@ SIGNATURES = {
     "hd_stats_gini": (C.c_int, [_U64, C.c_uint64, C.POINTER(C.c_double)]),
     "hd_stats_percentiles": (C.c_int, [_U64, C.c_uint64, C.c_char_p, C.c_uint64]),
     "hd_mbr_hd_bounds": (C.c_int, [_F, _F, C.c_int, _F, _F]),
+    "hd_hybrid_sample_iterations": (
+        C.c_int, [_P, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_int32,
+                  C.c_float, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint64)]),
     "hd_context_set_profiling": (C.c_int, [_P, C.c_int]),
     "hd_context_stage_times": (C.c_int, [_P, C.c_char_p, C.c_uint64, C.c_int]),
     "hd_device_count": (C.c_int, []),

@@ -645,3 +645,33 @@ int hd_nn_device(hd_context* ctx, int index, int n_dims, const float* d_q,
 }
 
 }  // extern "C"
+
+extern "C" int hd_hybrid_sample_iterations(hd_context* ctx, int n_dims, const float* d_a,
+                                           uint64_t n_a, const float* d_b, uint64_t n_b,
+                                           int32_t seed, float sample_rate,
+                                           uint32_t* counts, uint64_t cap, uint64_t* n_out) {
+  ClearError();
+  if (n_dims != 2 && n_dims != 3)
+    return Fail(HD_EINVAL, "n_dims must be 2 or 3, got %d", n_dims);
+  if (!d_a || !d_b || !n_out) return Fail(HD_EINVAL, "null pointer");
+  HD_TRY(ValidateSizes(n_a, n_b));
+  if (!(sample_rate >= 0.f && sample_rate <= 1.f))
+    return Fail(HD_EINVAL, "sample_rate must be in [0, 1]");
+  ScopedContext sc;
+  HD_TRY(AcquireContext(ctx, &sc));
+  DirectedArgs args;
+  args.n_dims = n_dims;
+  args.a = d_a;
+  args.n_a = n_a;
+  args.b = d_b;
+  args.n_b = n_b;
+  args.seed = seed;
+  args.sample_rate = sample_rate;
+  std::vector<uint32_t> v;
+  HD_TRY(SampledEbIterations(sc.ctx, args, &v));
+  *n_out = v.size();
+  if (counts) std::memcpy(counts, v.data(), std::min<uint64_t>(cap, v.size()) * 4);
+  if (counts && v.size() > cap)
+    return Fail(HD_ETRUNC, "%zu counts, room for %llu", v.size(), (unsigned long long)cap);
+  return HD_OK;
+}

@@ -179,6 +179,10 @@ int RunHybrid(hd_context* ctx, const DirectedArgs& args,
 // (no early break), then the maximum (nn.hip).
 int RunNearestNeighborSearch(hd_context* ctx, const DirectedArgs& args,
                              hd_directed_result* out);
+// The hybrid's sampled early-break pass (sample.hip): per sampled A point,
+// the reference's iteration count (units of 256 B points).
+int SampledEbIterations(hd_context* ctx, const DirectedArgs& args,
+                        std::vector<uint32_t>* counts);
 // The RT-HDIST baseline on a quantized grid (rthdist.hip).
 int RunRtHdist(hd_context* ctx, const DirectedArgs& args, hd_directed_result* out);
 // Grid query with early break whose queries that visit more than max_cells

@@ -11,6 +11,8 @@
 //  * otherwise the Morton grid with early break, queries visiting more than
 //    max_hit cells finished by the EB rounds (measured faster than the
 //    LBVH on the clustered 2D and the 3D-surface workloads);
+//  * eb_only_threshold > 0 and |B| < 500k: the reference's sampled pass
+//    decides between the early-break scan and the index path (sample.hip);
 //  * auto_tune: each candidate is run once on the actual input and the
 //    fastest is remembered for that input in the context (the reference
 //    trains its selector offline from a parameter sweep,
@@ -18,8 +20,10 @@
 // Every candidate returns the same bits; only the time differs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "hd_internal.h"
 
@@ -61,6 +65,22 @@ std::string Signature(const DirectedArgs& a) {
 
 int RunHybrid(hd_context* ctx, const DirectedArgs& args,
               hd_directed_result* out) {
+  // With an EB-only threshold and B below the reference's sample threshold
+  // (500k points, hausdorff_distance_hybrid.h:62,129), the sampled pass
+  // decides: the early-break scan alone when the sampled points' 80th
+  // percentile iteration count is below the threshold (:171-199), the
+  // index path otherwise.
+  if (args.eb_only_threshold > 0 && args.n_b < 500000u) {
+    std::vector<uint32_t> counts;
+    const int st = StageBegin(ctx, "hybrid_sample");
+    HD_TRY(SampledEbIterations(ctx, args, &counts));
+    StageEnd(ctx, st);
+    std::sort(counts.begin(), counts.end());
+    const uint32_t p80 = counts[(size_t)((double)counts.size() * 0.8)];
+    StageCount(ctx, "hybrid_sample", p80, counts.size());
+    return RunChoice(ctx, p80 < args.eb_only_threshold ? HD_VARIANT_EB : HD_VARIANT_GRID,
+                     args, out);
+  }
   if (!args.auto_tune) return RunChoice(ctx, RuleBased(args), args, out);
   const std::string sig = Signature(args);
   auto it = ctx->tuned_variant.find(sig);

@@ -253,6 +253,19 @@ int hd_stats_gini(const uint64_t* hist, uint64_t n_values, double* out);
  * each repeat of the stats JSON ("HDLowerBound", "HDUpperBound"). */
 int hd_mbr_hd_bounds(const float* mbr_a, const float* mbr_b, int n_dims,
                      float* lower, float* upper);
+/* The hybrid's sampled early-break pass (hausdorff_distance_hybrid.h:
+ * 120-199), used when eb_only_threshold > 0 and |B| < 500k: for the first
+ * ceil(|A| * sample_rate) points of A's pseudo-random order (seed), the
+ * iteration count of the reference's sampling kernel (units of 256 B
+ * points of a shuffled B, break checked every 32 iterations with <=
+ * against cmax2 = max(MBR lower bound^2, the sampled points' exact NN d2)).
+ * The hybrid runs the early-break scan alone when the 80th percentile
+ * (sorted counts, index floor(0.8 n)) is below eb_only_threshold. Device
+ * inputs; writes min(n, cap) counts, *n_out = n (HD_ETRUNC if n > cap). */
+int hd_hybrid_sample_iterations(hd_context* ctx, int n_dims, const float* d_a,
+                                uint64_t n_a, const float* d_b, uint64_t n_b,
+                                int32_t seed, float sample_rate, uint32_t* counts,
+                                uint64_t cap, uint64_t* n_out);
 int hd_stats_percentiles(const uint64_t* hist, uint64_t n_values, char* json,
                          uint64_t cap);
 

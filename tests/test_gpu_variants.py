@@ -164,3 +164,41 @@ def test_rt_hdist_matches_its_restatement(ctx, case):
     exact = oracle.brute_directed(a, b)
     assert (bits, arg) == (exact.cmax2_bits, exact.argmax)
     assert r.ab.nn == exact.nn
+
+
+def sample_counts(ctx, a, b, seed=0, rate=0.01):
+    import ctypes as C
+    import torch
+    from hausdorffdistance_amd import _lib
+
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    n = C.c_uint64()
+    buf = (C.c_uint32 * (len(a) + 1))()
+    rc = _lib.load().hd_hybrid_sample_iterations(
+        ctx.handle, a.shape[1], C.c_void_p(da.data_ptr()), len(a), C.c_void_p(db.data_ptr()),
+        len(b), seed, rate, buf, len(a) + 1, C.byref(n))
+    _lib.check(rc)
+    return np.array(buf[:n.value], np.uint32)
+
+
+def test_hybrid_sampled_eb_only_decision(ctx):
+    a = np.ascontiguousarray(hd.gen.uniform(121, 20000, 2))
+    b = np.ascontiguousarray(hd.gen.uniform(122, 15000, 2))
+    counts = sample_counts(ctx, a, b)
+    full = -(-len(b) // 256)
+    assert len(counts) == 200  # ceil(20000 * 0.01)
+    # counts are the reference's: 32k + 1 at the first check that breaks,
+    # or every iteration when none does
+    assert all(c == full or (c % 32 == 1 and c < full) for c in counts), counts
+    p80 = int(np.sort(counts)[int(len(counts) * 0.8)])
+    exact = oracle.brute_directed(a, b)
+    for thr, want in ((p80 + 1, hd.Variant.EB), (p80, hd.Variant.GRID),
+                      (1, hd.Variant.GRID), (10 ** 6, hd.Variant.EB)):
+        r = run(a, b, hd.Variant.HYBRID, symmetric=False, ctx=ctx, eb_only_threshold=thr)
+        assert r.ab.variant_used == want, (thr, p80)
+        assert (r.ab.cmax2_bits, r.ab.argmax, r.ab.nn) == (
+            exact.cmax2_bits, exact.argmax, exact.nn)
+    # B at one point: every sampled point is within cmax2 of the first B
+    # point, so every count is 1.
+    one = np.repeat(np.array([[0.5, 0.5]], np.float32), 3000, axis=0)
+    assert set(sample_counts(ctx, a, one, rate=0.05).tolist()) == {1}
