@@ -135,8 +135,9 @@ __global__ void MaxF32Kernel(const float* __restrict__ v, uint32_t n,
 }
 
 // Scan of one A point per wave: first k with dmin2 <= r_k^2, radius
-// r_{k+3}; the cells within it are walked in order (uniform across the
-// wave) and the lanes stride over each cell's points, then a wave minimum.
+// r_{k+3}; the cells of the box around it are tested 64 at a time (a lane
+// per cell), the non-empty ones within the radius are walked with the lanes
+// striding over each cell's points, then a wave minimum.
 template <int ND>
 __global__ __launch_bounds__(256) void RtHdistScanKernel(
     QGrid g, const float* __restrict__ a, uint32_t n_a,
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(256) void RtHdistScanKernel(
     unsigned long long* evals) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  unsigned long long my_evals = 0;
+  unsigned long long my_evals = 0, kmax = 0ull;
   for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n_a; i += n_waves) {
     const float dm = dmin2[i];
     uint32_t lo = 0, hi = n_radius - 4;  // radius[hi]^2 >= max dmin2
@@ -165,40 +166,84 @@ __global__ __launch_bounds__(256) void RtHdistScanKernel(
       p[d] = a[(uint64_t)i * ND + d];
     }
     int c0[3] = {0, 0, 0}, c1[3] = {0, 0, 0};
+    uint32_t w[3] = {1u, 1u, 1u};
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       c0[d] = (int)fmaxf(ceilf(q[d] - r), 0.f);
       c1[d] = (int)fminf(floorf(q[d] + r), (float)(g.res[d] - 1));
+      w[d] = c1[d] >= c0[d] ? (uint32_t)(c1[d] - c0[d] + 1) : 0u;
     }
+    const uint32_t n_box = w[0] * w[1] * (ND == 3 ? w[2] : 1u);
     float best = kFltMax;
-    for (int cz = (ND == 3 ? c0[2] : 0); cz <= (ND == 3 ? c1[2] : 0); ++cz) {
-      for (int cy = c0[1]; cy <= c1[1]; ++cy) {
-        for (int cx = c0[0]; cx <= c1[0]; ++cx) {
-          const float dx = (float)cx - q[0], dy = (float)cy - q[1];
-          float dd = dx * dx + dy * dy;
-          if constexpr (ND == 3) {
-            const float dz = (float)cz - q[2];
-            dd = dd + dz * dz;
-          }
-          if (!(dd <= r2)) continue;
+    // The box's cells 64 at a time: lane l takes cell base + l, tests its
+    // distance and loads its [start, end) (one round trip for 64 cells),
+    // then the wave walks the non-empty ones with the lanes on their points.
+    for (uint32_t base = 0; base < n_box; base += 64u) {
+      const uint32_t k = base + lane;
+      uint32_t cs = 0u, ce = 0u;
+      if (k < n_box) {
+        const int cx = c0[0] + (int)(k % w[0]);
+        const int cy = c0[1] + (int)((k / w[0]) % w[1]);
+        const int cz = ND == 3 ? c0[2] + (int)(k / (w[0] * w[1])) : 0;
+        const float dx = (float)cx - q[0], dy = (float)cy - q[1];
+        float dd = dx * dx + dy * dy;
+        if constexpr (ND == 3) {
+          const float dz = (float)cz - q[2];
+          dd = dd + dz * dz;
+        }
+        if (dd <= r2) {
           const uint32_t cell =
               (uint32_t)cx + (uint32_t)g.res[0] *
                                  ((uint32_t)cy + (ND == 3 ? (uint32_t)g.res[1] * (uint32_t)cz : 0u));
-          const uint32_t s = start[cell], e = end[cell];
-          for (uint32_t j = s + lane; j < e; j += 64u) {
-            const float d2 = ND == 2 ? D2(p[0], p[1], bx[j], by[j])
-                                     : D2(p[0], p[1], p[2], bx[j], by[j], bz[j]);
-            best = fminf(best, d2);
+          cs = start[cell];
+          ce = end[cell];
+        }
+      }
+      unsigned long long todo = __ballot(ce > cs);
+      while (todo != 0ull) {
+        // Four cells at a time, so that four independent loads are in
+        // flight per lane (one cell at a time waits a memory round trip per
+        // cell). An unused slot has s = e = 0 and reads index 0 unused.
+        uint32_t cs4[4], ce4[4], maxlen = 0u;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          cs4[c] = ce4[c] = 0u;
+          if (todo != 0ull) {
+            const int src = __builtin_ctzll(todo);
+            todo &= todo - 1ull;
+            cs4[c] = (uint32_t)__shfl(cs, src, 64);
+            ce4[c] = (uint32_t)__shfl(ce, src, 64);
           }
-          my_evals += e - s;
+          maxlen = max(maxlen, ce4[c] - cs4[c]);
+          my_evals += ce4[c] - cs4[c];
+        }
+        for (uint32_t off = lane; off < maxlen; off += 64u) {
+          float px[4], py[4], pz[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const uint32_t j = cs4[c] + off < ce4[c] ? cs4[c] + off : cs4[c];
+            px[c] = bx[j];
+            py[c] = by[j];
+            if constexpr (ND == 3) pz[c] = bz[j];
+          }
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            if (cs4[c] + off < ce4[c]) {
+              const float d2 = ND == 2 ? D2(p[0], p[1], px[c], py[c])
+                                       : D2(p[0], p[1], p[2], px[c], py[c], pz[c]);
+              best = fminf(best, d2);
+            }
+          }
         }
       }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) best = fminf(best, __shfl_xor(best, off, 64));
-    if (lane == 0 && best != kFltMax) atomicMax(key, PackKey(best, i));
+    if (best != kFltMax) kmax = max(kmax, PackKey(best, i));
   }
-  // my_evals is the same in every lane of a wave: one lane counts it.
+  // One atomic per wave (an atomic per A point on the one key serialises:
+  // 1M of them cost ~10 ms). kmax and my_evals are the same in every lane.
+  if (lane == 0 && kmax != 0ull) atomicMax(key, kmax);
   if (lane == 0 && my_evals) atomicAdd(evals, my_evals);
 }
 
