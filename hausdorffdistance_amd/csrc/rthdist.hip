@@ -123,15 +123,20 @@ __global__ void QuantizeAKernel(QGrid g, const float* __restrict__ a,
   }
 }
 
-__global__ void MaxF32Kernel(const float* __restrict__ v, uint32_t n,
-                             unsigned int* out_bits) {
+// Maximum of non-negative floats as bits; one atomic per block (256 threads).
+__global__ __launch_bounds__(256) void MaxF32Kernel(const float* __restrict__ v, uint32_t n,
+                                                    unsigned int* out_bits) {
+  __shared__ float part[4];
   float m = 0.f;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x)
     m = fmaxf(m, v[i]);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-  if ((threadIdx.x & 63u) == 0) atomicMax(out_bits, __float_as_uint(m));
+  if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicMax(out_bits, __float_as_uint(fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]))));
 }
 
 // Scan of one A point per wave: first k with dmin2 <= r_k^2, radius
@@ -381,7 +386,8 @@ int RunRtHdist(hd_context* ctx, const DirectedArgs& args, hd_directed_result* ou
   HD_TRY(ContextArray(ctx, "rth_nnidx", n_a, &nn_idx));
   HD_TRY(GridNearestNeighbors(ctx, nd, qa, n_a, reps, h_reps, dmin2, nn_idx));
   HD_TRY(FillU32(ctx, n_reps + 1, 0u, 1));
-  MaxF32Kernel<<<std::min(ab, 2048), 256, 0, ctx->stream>>>(dmin2, n_a, n_reps + 1);
+  MaxF32Kernel<<<std::min(ab, std::max(1, ctx->cu_count) * 2), 256, 0, ctx->stream>>>(
+      dmin2, n_a, n_reps + 1);
   HD_HIP_TRY(hipGetLastError());
   unsigned int max_bits = 0;
   HD_HIP_TRY(hipMemcpyAsync(&max_bits, n_reps + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
