@@ -1409,6 +1409,11 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
   const uint32_t* list = order;
   const float* list_min = order_min;
   float cmax2 = 0.f;
+  // cmax2 holds the key's value as of the last launch that could raise it
+  // (read together with the escalation count; only top completions change
+  // the key between that read and the next round), so the round start and
+  // the tail read it again only after top completions.
+  bool cmax2_current = false;
   // A pseudo-random order of a large set ranks on a sample: the first round
   // (short budget, cmax2 still far from its final value, exact scan) only
   // visits the first eighth of the order, enough for its top completions
@@ -1425,7 +1430,7 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
     const uint32_t budget = first_budget ? first_budget : kBudgets[r];
     const bool with_extra = r == 1 && extra_begin < extra_end;
     rs->round_points[r] += end - begin;
-    if (t.bt != nullptr && budget < t.n_steps)
+    if (t.bt != nullptr && budget < t.n_steps && !cmax2_current)
       HD_TRY(ReadCount(ctx, nullptr, nullptr, key, &cmax2));
     // Matrix-core filter when its threshold keeps most of cmax2, else the
     // exact scan.
@@ -1460,6 +1465,7 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
     }
     uint32_t n_esc = 0;
     HD_TRY(ReadCount(ctx, counters + 1, &n_esc, key, &cmax2));
+    cmax2_current = true;
     if (n_esc == 0) {
       begin = end = 0;
       continue;
@@ -1487,6 +1493,7 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
         {"eb_mfma_top", "eb_top_exact", "eb_top"}, &top_filt,
         &rs->top_exact_points)));
     if (top_filt) rs->top_filtered_points += k_top;
+    if (k_top > 0) cmax2_current = false;
     // The sorted escalations become the next round's list; the next round
     // writes its sort into the buffer this round read from.
     mode = kOrderArray;
@@ -1508,7 +1515,7 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
     rs->tail_points += end - begin;
     const uint32_t n_tail = end - begin;
     const float* tail_min = list_min ? list_min + begin : nullptr;
-    if (t.bt != nullptr && t.n_steps >= 4096u)
+    if (t.bt != nullptr && t.n_steps >= 4096u && !cmax2_current)
       HD_TRY(ReadCount(ctx, nullptr, nullptr, key, &cmax2));
     HD_TRY((CompleteEntries<ND>(
         ctx, a, list + begin, tail_min, n_tail, t, cmax2, key,
