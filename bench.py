@@ -324,6 +324,24 @@ def secondary(args) -> None:
         results[name] = {"ms_per_step": ms_v, "a_points_per_s": a.shape[0] / (ms_v / 1e3),
                          "evals": r.evals,
                          "stages_ms": {k: x["ms"] / max(1, x["calls"]) for k, x in st_v.items()}}
+    if "eb" in results and args.workload == "gmm_50m_256_2d" and not args.no_learned:
+        # The learned order on the larger mixture: the model trained on a
+        # 1M-point subsample of A (exact NN labels), then the same EB run.
+        t0 = time.perf_counter()
+        model = hd.Model.train(ctx, da, db, steps=2000, batch=4096, n_samples=1_000_000)
+        torch.cuda.synchronize()
+        train_s = time.perf_counter() - t0
+        ms_l, res_l, st_l = time_steps(hd, ctx, hd.Variant.EB, da, db, args.steps, args.warmup,
+                                       learned=True, model=model)
+        r = res_l[-1]
+        if (r.cmax2_bits, r.argmax, r.nn) != ref:
+            raise RuntimeError("learned order changed the result")
+        log(f"eb learned: {ms_l:.3f} ms/step, evals {r.evals}, train {train_s:.2f} s")
+        results["eb_learned"] = {
+            "ms_per_step": ms_l, "a_points_per_s": a.shape[0] / (ms_l / 1e3), "evals": r.evals,
+            "model_train_s": train_s,
+            "speedup_vs_plain_eb": results["eb"]["ms_per_step"] / ms_l,
+            "stages_ms": {k: x["ms"] / max(1, x["calls"]) for k, x in st_l.items()}}
     best = min(results, key=lambda k: results[k]["ms_per_step"])
     ms = results[best]["ms_per_step"]
     line = {"metric": f"A-points/sec, {args.workload} directed HD", "value": a.shape[0] / (ms / 1e3),
