@@ -17,6 +17,10 @@ config.next_rows, the input-statistics pipeline, the RT-HDIST baseline (on
 1M x 1M uniform 2D) and the loaders' parse rates are timed beside it; none
 of them enters `value`.
 
+Secondary workloads (--workload): uniform_100k_2d, sphere_20m_15m_3d,
+batch_4096_2d, and gmm_50m_256_2d, which also times the learned order
+against the plain one.
+
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
 from __future__ import annotations
