@@ -24,6 +24,7 @@ from .api import (  # noqa: F401
     device_count,
     device_info,
     directed_device,
+    directed_host,
     mlp_loss_grad,
     nn_device,
     run_hausdorff_distance,

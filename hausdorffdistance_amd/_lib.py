@@ -35,6 +35,7 @@ HD_FLAG_LEARNED_ORDER = 0x4
 HD_FLAG_COUNT_EVALS = 0x8
 HD_FLAG_INPUT_ON_DEVICE = 0x10
 HD_FLAG_NO_SHUFFLE = 0x20
+HD_FLAG_SINGLE_PASS = 0x40
 
 HD_ABI_VERSION = 1
 
@@ -122,6 +123,11 @@ SIGNATURES = {
     "hd_directed_device": (
         C.c_int,
         [_P, C.c_int, C.c_int, _P, C.c_uint64, _P, C.c_uint64, C.c_uint32, C.c_int32, _P,
+         C.POINTER(DirectedResultC)],
+    ),
+    "hd_directed_host": (
+        C.c_int,
+        [C.c_int, _P, C.c_uint64, _P, C.c_uint64, C.c_uint32, C.c_int32, C.c_int32,
          C.POINTER(DirectedResultC)],
     ),
     "hd_nn_device": (

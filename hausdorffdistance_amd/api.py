@@ -61,8 +61,10 @@ class Execution(IntEnum):
 @dataclass
 class RunConfig:
     """Reference RunConfig restricted to the hot path; defaults of
-    src/flags.cc:3-57 (check defaults to False here because the reference's
-    check re-runs its CPU path, which this library does not serve)."""
+    src/flags.cc:3-57, except `check`: the reference's -check defaults to on
+    (flags.cc:49) and re-runs its CPU early break after every run; here it is
+    opt-in for library callers (the hd_exec CLI keeps the reference's
+    default)."""
 
     variant: Variant = Variant.EB
     execution: Execution = Execution.GPU
@@ -229,11 +231,14 @@ def run_hausdorff_distance(config: RunConfig, points_a, points_b,
         c.flags |= _lib.HD_FLAG_INPUT_ON_DEVICE
     out = _lib.ResultC()
     buf = C.create_string_buffer(stats_cap)
-    own = ctx is None
-    ctx = ctx or Context(0)
+    # CPU execution touches no GPU: no context is made for it.
+    on_cpu = config.execution == Execution.CPU
+    own = ctx is None and not on_cpu
+    if own:
+        ctx = Context(0)
     try:
-        rc = lib.hd_run_hausdorff_distance(ctx.handle, C.byref(c), pa, na, pb, nb,
-                                           C.byref(out), buf, stats_cap)
+        rc = lib.hd_run_hausdorff_distance(ctx.handle if ctx is not None else None, C.byref(c),
+                                           pa, na, pb, nb, C.byref(out), buf, stats_cap)
         _lib.check(rc)
     finally:
         if own:
@@ -246,8 +251,11 @@ def run_hausdorff_distance(config: RunConfig, points_a, points_b,
 
 def directed_device(ctx: Context, variant: Variant, points_a, points_b, *,
                     n_dims: Optional[int] = None, seed: int = 0, shuffle: bool = True,
-                    learned_order: bool = False, model=None) -> DirectedResult:
-    """h(A -> B) on device-resident (torch CUDA) inputs."""
+                    learned_order: bool = False, model=None,
+                    single_pass: bool = False) -> DirectedResult:
+    """h(A -> B) on device-resident (torch CUDA) inputs. single_pass (early
+    break only): one pass over A in the visit order with no seed, rounds or
+    filter, the reference GPU kernel's schedule (a comparison baseline)."""
     nd = n_dims or int(points_a.shape[1])
     pa, na, da, _ = _points(points_a, nd)
     pb, nb, db, _ = _points(points_b, nd)
@@ -258,10 +266,30 @@ def directed_device(ctx: Context, variant: Variant, points_a, points_b, *,
         flags |= _lib.HD_FLAG_NO_SHUFFLE
     if learned_order:
         flags |= _lib.HD_FLAG_LEARNED_ORDER
+    if single_pass:
+        flags |= _lib.HD_FLAG_SINGLE_PASS
     out = _lib.DirectedResultC()
     mh = getattr(model, "handle", None)
     _lib.check(_lib.load().hd_directed_device(ctx.handle, int(variant), nd, pa, na, pb, nb,
                                               flags, seed, mh, C.byref(out)))
+    return DirectedResult.from_c(out)
+
+
+def directed_host(points_a, points_b, *, n_dims: Optional[int] = None, seed: int = 0,
+                  shuffle: bool = True, parallelism: int = -1) -> DirectedResult:
+    """h(A -> B) on the host CPU (the reference's CPU overload,
+    early_break.h:44-115): host arrays, `parallelism` threads (<= 0: all
+    hardware threads). Same bits, arg-max and NN as the GPU variants."""
+    a = np.asarray(points_a)
+    nd = n_dims or int(a.shape[1])
+    pa, na, da, ka = _points(points_a, nd)
+    pb, nb, db, kb = _points(points_b, nd)
+    if da or db:
+        raise HDError(_lib.HD_EINVAL, "directed_host takes host arrays")
+    flags = 0 if shuffle else _lib.HD_FLAG_NO_SHUFFLE
+    out = _lib.DirectedResultC()
+    _lib.check(_lib.load().hd_directed_host(nd, pa, na, pb, nb, flags, seed, parallelism,
+                                            C.byref(out)))
     return DirectedResult.from_c(out)
 
 
@@ -279,22 +307,41 @@ class HausdorffDistance:
     """Mirror of hd::HausdorffDistance<COORD_T, N_DIMS>
     (src/hausdorff_distance.h:15-42): calculate_distance returns sqrt(cmax2)
     as a float and fills get_stats() with Algorithm / Execution /
-    ReportedTime. Inputs are torch CUDA tensors; unlike the reference they are
-    not shuffled in place."""
+    ReportedTime. The two overloads of the reference are told apart by where
+    the inputs live: torch CUDA tensors run on the GPU (:27-32), host arrays
+    on the CPU (:21-26; only the early break has a CPU path, the other
+    variants raise HD_EUNSUPPORTED as the reference aborts). Unlike the
+    reference the inputs are not shuffled in place."""
 
     variant = Variant.EB
     algorithm = "Early Break"
 
-    def __init__(self, ctx: Optional[Context] = None, seed: int = 0, **_: Any):
-        self.ctx = ctx or Context(0)
+    def __init__(self, ctx: Optional[Context] = None, seed: int = 0, n_threads: int = -1,
+                 **_: Any):
+        self._ctx = ctx
         self.seed = seed
+        self.n_threads = n_threads
         self._stats: dict = {}
         self.last: Optional[DirectedResult] = None
 
+    @property
+    def ctx(self) -> Context:
+        if self._ctx is None:
+            self._ctx = Context(0)
+        return self._ctx
+
     def calculate_distance(self, points_a, points_b) -> float:
-        r = directed_device(self.ctx, self.variant, points_a, points_b, seed=self.seed)
+        if _is_torch_cuda(points_a) or _is_torch_cuda(points_b):
+            r = directed_device(self.ctx, self.variant, points_a, points_b, seed=self.seed)
+            execution = "GPU"
+        elif self.variant == Variant.EB:
+            r = directed_host(points_a, points_b, seed=self.seed, parallelism=self.n_threads)
+            execution = "CPU"
+        else:
+            raise HDError(_lib.HD_EUNSUPPORTED,
+                          f"CalculateDistance on the CPU is not implemented for {self.algorithm}")
         self.last = r
-        self._stats = {"Algorithm": self.algorithm, "Execution": "GPU",
+        self._stats = {"Algorithm": self.algorithm, "Execution": execution,
                        "ReportedTime": r.time_ms, "ComparedPairs": r.evals,
                        "CMax2Bits": r.cmax2_bits, "ArgMax": r.argmax, "NN": r.nn}
         return r.hd
@@ -345,10 +392,23 @@ def batch_directed(ctx: Context, points_a, offs_a, points_b, offs_b) -> list:
     points (N, d) float32, offsets (n_pairs + 1,) int64 in points). One
     persistent-kernel launch replayed from a cached HIP graph."""
     nd = int(points_a.shape[1])
-    pa, _, da, _ = _points(points_a, nd)
-    pb, _, db, _ = _points(points_b, nd)
+    pa, n_pts_a, da, _ = _points(points_a, nd)
+    pb, n_pts_b, db, _ = _points(points_b, nd)
     if not (da and db):
         raise HDError(_lib.HD_EINVAL, "batch_directed needs CUDA tensors")
+    for name, offs, n_pts in (("offs_a", offs_a, n_pts_a), ("offs_b", offs_b, n_pts_b)):
+        if not _is_torch_cuda(offs) or str(offs.dtype) != "torch.int64" or offs.dim() != 1:
+            raise HDError(_lib.HD_EINVAL, f"{name} must be a 1-D int64 CUDA tensor")
+        if not offs.is_contiguous():
+            raise HDError(_lib.HD_EINVAL, f"{name} must be contiguous")
+        if offs.shape[0] < 2:
+            raise HDError(_lib.HD_EINVAL, f"{name} needs n_pairs + 1 >= 2 entries")
+        first, last = (int(v) for v in offs[[0, -1]].tolist())
+        if first < 0 or last > n_pts:
+            raise HDError(_lib.HD_EINVAL,
+                          f"{name} spans [{first}, {last}) outside the {n_pts} points")
+    if offs_a.shape[0] != offs_b.shape[0]:
+        raise HDError(_lib.HD_EINVAL, "offs_a and offs_b must have the same length")
     n_pairs = int(offs_a.shape[0]) - 1
     out = (_lib.DirectedResultC * max(n_pairs, 1))()
     _lib.check(_lib.load().hd_batch_directed_device(

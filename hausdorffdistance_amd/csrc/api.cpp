@@ -130,6 +130,144 @@ int TimedDirected(hd_context* ctx, int variant, const DirectedArgs& args,
   return HD_OK;
 }
 
+// Check (run_hausdorff_distance.cu:366-388): the host early break with
+// std::thread::hardware_concurrency() threads (:373) on host inputs; passes
+// when cmax2 bits and arg-max are those of `out` in every direction run.
+int CheckOnHost(const hd_run_config& cfg, const float* a, uint64_t n_a,
+                const float* b, uint64_t n_b, hd_result* out,
+                std::string* check_json) {
+  const int nd = cfg.n_dims;
+  const bool symmetric = (cfg.flags & HD_FLAG_SYMMETRIC) != 0;
+  const uint32_t flags = cfg.flags & HD_FLAG_NO_SHUFFLE;
+  hd_directed_result rab, rba;
+  std::memset(&rba, 0, sizeof(rba));
+  HD_TRY(CpuEarlyBreak(nd, a, n_a, b, n_b, flags, cfg.seed, 0, &rab, nullptr));
+  if (symmetric)
+    HD_TRY(CpuEarlyBreak(nd, b, n_b, a, n_a, flags, cfg.seed, 0, &rba, nullptr));
+  const bool pass = rab.cmax2_bits == out->ab.cmax2_bits &&
+                    rab.argmax == out->ab.argmax &&
+                    (!symmetric || (rba.cmax2_bits == out->ba.cmax2_bits &&
+                                    rba.argmax == out->ba.argmax));
+  const hd_directed_result& cw =
+      (symmetric && rba.cmax2_bits > rab.cmax2_bits) ? rba : rab;
+  out->check_pass = pass ? 1 : 0;
+  *check_json = ", \"Check\": {\"HDAnswer\": " + Num(cw.hd) +
+                ", \"Diff\": " + Num((double)cw.hd - (double)out->hd) +
+                ", \"Pass\": " + (pass ? "true" : "false") + "}";
+  return HD_OK;
+}
+
+// execution = CPU: the early-break variant on the host (the reference's CPU
+// overload, early_break.h:44-115), `repeat` times, no GPU touched. Other
+// variants have no CPU overload in the reference either
+// (hausdorff_distance.h:21-26).
+int RunOnHost(const hd_run_config& cfg, const float* a, uint64_t n_a,
+              const float* b, uint64_t n_b, hd_result* out, char* stats_json,
+              uint64_t stats_cap) {
+  if (cfg.variant != HD_VARIANT_EB)
+    return Fail(HD_EUNSUPPORTED,
+                "CalculateDistance on the CPU is not implemented for %s",
+                AlgorithmName(cfg.variant));
+  if (cfg.flags & HD_FLAG_INPUT_ON_DEVICE)
+    return Fail(HD_EINVAL, "CPU execution takes host inputs");
+  if (cfg.flags & HD_FLAG_LEARNED_ORDER)
+    return Fail(HD_EUNSUPPORTED,
+                "the learned visit order needs the GPU (execution = GPU)");
+  const int nd = cfg.n_dims;
+  const bool symmetric = (cfg.flags & HD_FLAG_SYMMETRIC) != 0;
+  const uint32_t flags = cfg.flags & HD_FLAG_NO_SHUFFLE;
+  std::vector<double> times;
+  std::ostringstream reps;
+  for (int r = 0; r < cfg.repeat; ++r) {
+    hd_directed_result rab, rba;
+    std::memset(&rba, 0, sizeof(rba));
+    CpuRunInfo iab, iba;
+    HD_TRY(CpuEarlyBreak(nd, a, n_a, b, n_b, flags, cfg.seed, cfg.parallelism,
+                         &rab, &iab));
+    double t = rab.time_ms, shuffle = iab.shuffle_ms;
+    if (symmetric) {
+      HD_TRY(CpuEarlyBreak(nd, b, n_b, a, n_a, flags, cfg.seed,
+                           cfg.parallelism, &rba, &iba));
+      t += rba.time_ms;
+      shuffle += iba.shuffle_ms;
+    }
+    times.push_back(t);
+    out->ab = rab;
+    out->ba = rba;
+    if (r) reps << ", ";
+    reps << "{\"Repeat\": " << (r + 1)
+         << ", \"Algorithm\": \"Early Break\", \"Execution\": \"CPU\""
+         << ", \"Threads\": " << iab.threads << ", \"SIMD\": \"" << iab.simd
+         << "\", \"ComparedPairs\": " << (rab.evals + rba.evals)
+         << ", \"ShuffleTime\": " << Num(shuffle)
+         << ", \"ComputeTime\": " << Num(t) << ", \"ReportedTime\": " << Num(t)
+         << ", \"CMax2Bits\": " << rab.cmax2_bits << ", \"ArgMax\": "
+         << rab.argmax << "}";
+  }
+  out->n_repeats = (uint32_t)cfg.repeat;
+  const bool ba_wins = symmetric && out->ba.cmax2_bits > out->ab.cmax2_bits;
+  const hd_directed_result& win = ba_wins ? out->ba : out->ab;
+  out->hd = win.hd;
+  out->cmax2_bits = win.cmax2_bits;
+  out->argmax = win.argmax;
+  out->nn = win.nn;
+  out->direction = ba_wins ? 1 : 0;
+  std::vector<double> sorted = times;
+  std::sort(sorted.begin(), sorted.end());
+  double sum = 0;
+  for (double t : times) sum += t;
+  out->avg_time_ms = sum / times.size();
+  out->min_time_ms = sorted.front();
+  out->median_time_ms = sorted[sorted.size() / 2];
+
+  std::string check_json;
+  if (cfg.flags & HD_FLAG_CHECK)
+    HD_TRY(CheckOnHost(cfg, a, n_a, b, n_b, out, &check_json));
+
+  if (stats_json && stats_cap > 0) {
+    // Input object from host MBRs (the statistics grid needs the GPU and is
+    // left out on this path).
+    float mbr_a[6], mbr_b[6], mbr_ab[6], lo_a[3], hi_a[3], lo_b[3], hi_b[3],
+        lo[3], hi[3];
+    HostPointsMbr(nd, a, n_a, mbr_a);
+    HostPointsMbr(nd, b, n_b, mbr_b);
+    NonEmptyMbr(nd, mbr_a, lo_a, hi_a);
+    NonEmptyMbr(nd, mbr_b, lo_b, hi_b);
+    for (int d = 0; d < nd; ++d) {
+      mbr_ab[d] = std::min(mbr_a[d], mbr_b[d]);
+      mbr_ab[nd + d] = std::max(mbr_a[nd + d], mbr_b[nd + d]);
+      lo[d] = std::min(lo_a[d], lo_b[d]);
+      hi[d] = std::max(hi_a[d], hi_b[d]);
+    }
+    const float density = (float)(n_a + n_b) / MbrVolume(nd, lo, hi);
+    const StatsInput sets[2] = {{a, n_a}, {b, n_b}};
+    std::string files[2], both;
+    HD_TRY(InputStatsJson(nullptr, nd, &sets[0], 1, mbr_a, 0, nullptr, &files[0]));
+    HD_TRY(InputStatsJson(nullptr, nd, &sets[1], 1, mbr_b, 0, nullptr, &files[1]));
+    HD_TRY(InputStatsJson(nullptr, nd, sets, 2, mbr_ab, 0, &density, &both));
+    both.pop_back();
+    std::ostringstream js;
+    js << "{\"Input\": " << both << ", \"Files\": [" << files[0] << ", "
+       << files[1] << "], \"NumDims\": " << nd << ", \"Type\": \"Float\"}"
+       << ", \"Running\": {\"Seed\": " << cfg.seed
+       << ", \"Parallelism\": " << HostThreads(cfg.parallelism)
+       << ", \"Symmetric\": " << (symmetric ? "true" : "false")
+       << ", \"Repeats\": [" << reps.str()
+       << "], \"AvgTime\": " << Num(out->avg_time_ms)
+       << ", \"MedianTime\": " << Num(out->median_time_ms) << "}"
+       << ", \"HDResult\": " << Num(out->hd)
+       << ", \"CMax2Bits\": " << out->cmax2_bits
+       << ", \"ArgMax\": " << out->argmax << ", \"NN\": " << out->nn
+       << ", \"Direction\": " << out->direction << check_json << "}";
+    const std::string s = js.str();
+    std::strncpy(stats_json, s.c_str(), stats_cap - 1);
+    stats_json[stats_cap - 1] = '\0';
+    if (s.size() + 1 > stats_cap)
+      return Fail(HD_ETRUNC, "stats JSON needs %zu bytes", s.size() + 1);
+  }
+  return HD_OK;
+}
+
 }  // namespace
 
 void SetError(const std::string& msg) { g_last_error = msg; }
@@ -391,6 +529,8 @@ int hd_directed_device(hd_context* ctx, int variant, int n_dims,
     return Fail(HD_EINVAL, "n_dims must be 2 or 3, got %d", n_dims);
   if (!ValidVariant(variant))
     return Fail(HD_EINVAL, "unknown variant %d", variant);
+  if ((flags & HD_FLAG_SINGLE_PASS) && variant != HD_VARIANT_EB)
+    return Fail(HD_EINVAL, "SINGLE_PASS applies to the early-break variant");
   HD_TRY(ValidateSizes(n_a, n_b));
   if (!d_a || !d_b) return Fail(HD_EINVAL, "null point array");
   ScopedContext sc;
@@ -432,13 +572,12 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
     return Fail(HD_EINVAL, "Unknown variant: %d", cfg->variant);
   if (cfg->execution != HD_EXEC_GPU && cfg->execution != HD_EXEC_CPU)
     return Fail(HD_EINVAL, "Unknown execution: %d", cfg->execution);
-  if (cfg->execution == HD_EXEC_CPU)
-    return Fail(HD_EUNSUPPORTED,
-                "CPU execution is not served by this library (GPU only)");
   if (cfg->repeat < 1)
     return Fail(HD_EINVAL, "repeat must be >= 1, got %d", cfg->repeat);
   HD_TRY(ValidateSizes(n_a, n_b));
   if (!a || !b) return Fail(HD_EINVAL, "null point array");
+  if (cfg->execution == HD_EXEC_CPU)
+    return RunOnHost(*cfg, a, n_a, b, n_b, out, stats_json, stats_cap);
   const int nd = cfg->n_dims;
 
   ScopedContext sc;
@@ -539,27 +678,24 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
   out->min_time_ms = sorted.front();
   out->median_time_ms = sorted[sorted.size() / 2];
 
-  // Check (reference run_hausdorff_distance.cu:366-388 re-runs the CPU EB and
-  // compares with ==; here the plain GPU EB is the independent path).
+  // Check (reference run_hausdorff_distance.cu:366-388): the host early
+  // break with hardware_concurrency threads on the same inputs.
   std::string check_json;
   if (cfg->flags & HD_FLAG_CHECK) {
-    DirectedArgs cab = ab, cba = ba;
-    cab.flags &= ~(HD_FLAG_LEARNED_ORDER);
-    cba.flags &= ~(HD_FLAG_LEARNED_ORDER);
-    hd_directed_result rab, rba;
-    std::memset(&rba, 0, sizeof(rba));
-    HD_TRY(Dispatch(c, HD_VARIANT_EB, cab, &rab));
-    if (symmetric) HD_TRY(Dispatch(c, HD_VARIANT_EB, cba, &rba));
-    const bool pass = rab.cmax2_bits == out->ab.cmax2_bits &&
-                      rab.argmax == out->ab.argmax &&
-                      (!symmetric || (rba.cmax2_bits == out->ba.cmax2_bits &&
-                                      rba.argmax == out->ba.argmax));
-    const hd_directed_result& cw =
-        (symmetric && rba.cmax2_bits > rab.cmax2_bits) ? rba : rab;
-    out->check_pass = pass ? 1 : 0;
-    check_json = ", \"Check\": {\"HDAnswer\": " + Num(cw.hd) +
-                 ", \"Diff\": " + Num((double)cw.hd - (double)out->hd) +
-                 ", \"Pass\": " + (pass ? "true" : "false") + "}";
+    const float* ha = a;
+    const float* hb = b;
+    std::vector<float> copy_a, copy_b;
+    if (cfg->flags & HD_FLAG_INPUT_ON_DEVICE) {
+      copy_a.resize(n_a * nd);
+      copy_b.resize(n_b * nd);
+      HD_HIP_TRY(hipMemcpy(copy_a.data(), d_a, n_a * nd * sizeof(float),
+                           hipMemcpyDeviceToHost));
+      HD_HIP_TRY(hipMemcpy(copy_b.data(), d_b, n_b * nd * sizeof(float),
+                           hipMemcpyDeviceToHost));
+      ha = copy_a.data();
+      hb = copy_b.data();
+    }
+    HD_TRY(CheckOnHost(*cfg, ha, n_a, hb, n_b, out, &check_json));
   }
 
   if (stats_json && stats_cap > 0) {

@@ -290,6 +290,10 @@ __global__ void ZeroKernel(unsigned long long* keys, uint32_t n_keys,
 // Cached plan + graph for one batch signature.
 struct BatchPlan {
   std::string sig;
+  // The offsets the plan was built for: pair boundaries are baked into the
+  // work items and the graph, so a call whose offsets differ (the same
+  // device addresses holding another split) needs a new plan.
+  std::vector<uint64_t> offs_a, offs_b;
   int n_dims = 2;
   uint32_t n_pairs = 0;
   uint32_t n_items = 0;
@@ -310,15 +314,11 @@ void DestroyBatchPlan(BatchPlan* p) {
 
 namespace {
 
+// oa, ob: the offsets (host copies, n_pairs + 1 each).
 template <int ND>
-int BuildPlan(hd_context* ctx, const float* d_a, const uint64_t* d_offs_a,
-              const float* d_b, const uint64_t* d_offs_b, uint32_t n_pairs,
-              uint32_t chunk, BatchPlan* plan) {
-  std::vector<uint64_t> oa(n_pairs + 1), ob(n_pairs + 1);
-  HD_HIP_TRY(hipMemcpy(oa.data(), d_offs_a, sizeof(uint64_t) * (n_pairs + 1),
-                       hipMemcpyDeviceToHost));
-  HD_HIP_TRY(hipMemcpy(ob.data(), d_offs_b, sizeof(uint64_t) * (n_pairs + 1),
-                       hipMemcpyDeviceToHost));
+int BuildPlan(hd_context* ctx, const float* d_a, const std::vector<uint64_t>& oa,
+              const float* d_b, const std::vector<uint64_t>& ob,
+              uint32_t n_pairs, uint32_t chunk, BatchPlan* plan) {
   std::vector<BatchPair> pairs(n_pairs);
   uint64_t soa = 0;
   for (uint32_t p = 0; p < n_pairs; ++p) {
@@ -439,20 +439,37 @@ extern "C" int hd_batch_directed_device(hd_context* ctx, int n_dims,
     return Fail(HD_EUNSUPPORTED, "batch mode takes COUNT_EVALS only");
   HD_TRY(ContextActivate(ctx));
   char sig[192];
-  snprintf(sig, sizeof(sig), "%d:%p:%p:%p:%p:%u", n_dims, (const void*)d_a,
-           (const void*)d_offs_a, (const void*)d_b, (const void*)d_offs_b,
-           n_pairs);
+  snprintf(sig, sizeof(sig), "%d:%p:%p:%u", n_dims, (const void*)d_a,
+           (const void*)d_b, n_pairs);
+  // The offsets are read on every call (2 x (n_pairs + 1) x 8 bytes): the
+  // cached plan is reused only for the same point arrays and the same split.
+  std::vector<uint64_t> oa(n_pairs + 1), ob(n_pairs + 1);
+  {
+    void* pinned = nullptr;
+    const size_t bytes = sizeof(uint64_t) * (n_pairs + 1);
+    HD_TRY(ContextPinned(ctx, 2 * bytes, &pinned));
+    char* h = static_cast<char*>(pinned);
+    HD_HIP_TRY(hipMemcpyAsync(h, d_offs_a, bytes, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    HD_HIP_TRY(hipMemcpyAsync(h + bytes, d_offs_b, bytes,
+                              hipMemcpyDeviceToHost, ctx->stream));
+    HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::memcpy(oa.data(), h, bytes);
+    std::memcpy(ob.data(), h + bytes, bytes);
+  }
   BatchPlan* plan = ctx->batch_plan;
-  if (plan == nullptr || plan->sig != sig) {
+  if (plan == nullptr || plan->sig != sig || plan->offs_a != oa ||
+      plan->offs_b != ob) {
     if (plan) DestroyBatchPlan(plan);
     ctx->batch_plan = nullptr;
     auto build = [&](uint32_t chunk, BatchPlan** out_plan) -> int {
       BatchPlan* p = new BatchPlan();
       p->sig = sig;
-      const int rc = n_dims == 2 ? BuildPlan<2>(ctx, d_a, d_offs_a, d_b,
-                                                d_offs_b, n_pairs, chunk, p)
-                                 : BuildPlan<3>(ctx, d_a, d_offs_a, d_b,
-                                                d_offs_b, n_pairs, chunk, p);
+      p->offs_a = oa;
+      p->offs_b = ob;
+      const int rc = n_dims == 2
+                         ? BuildPlan<2>(ctx, d_a, oa, d_b, ob, n_pairs, chunk, p)
+                         : BuildPlan<3>(ctx, d_a, oa, d_b, ob, n_pairs, chunk, p);
       if (rc != HD_OK) {
         DestroyBatchPlan(p);
         return rc;

@@ -176,6 +176,8 @@ Feistel MakeFeistel(uint32_t n, uint64_t seed, uint32_t stream_id) {
   return f;
 }
 
+uint32_t HostPermute(const Feistel& f, uint32_t i) { return dev::Permute(f, i); }
+
 int FillU32(hd_context* ctx, uint32_t* p, uint32_t value, uint64_t n) {
   if (n == 0) return HD_OK;
   FillU32Kernel<<<GridFor(n, 256, 4096), 256, 0, ctx->stream>>>(p, value, n);

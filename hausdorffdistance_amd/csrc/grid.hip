@@ -377,11 +377,30 @@ int BuildGrid(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
   std::memset(&g, 0, sizeof(g));
   // Resolution: reference uniform_grid.h:222-244 (cell side from the
   // volume and points per cell, dims = ceil(extent / side)).
-  double volume = 1;
-  for (int d = 0; d < nd; ++d) volume *= (double)(mbr[nd + d] - mbr[d]);
+  // An axis whose extent is zero (a flat set, or one whose +1e-5 widening
+  // rounded away at large coordinates) gets one cell, and the side comes
+  // from the other axes' volume in their own dimension; otherwise a zero
+  // volume would give s = 0, which no growth below can leave.
+  double volume = 1, max_ext = 0;
+  int k = 0;
+  for (int d = 0; d < nd; ++d) {
+    const double ext = (double)(mbr[nd + d] - mbr[d]);
+    max_ext = std::max(max_ext, ext);
+    if (ext > 0) {
+      volume *= ext;
+      ++k;
+    }
+  }
   const double npc = n_points_cell ? n_points_cell : 8;
-  double s = nd == 2 ? std::sqrt(volume * npc / (double)n_b)
-                     : std::cbrt(volume * npc / (double)n_b);
+  double s = 1.0;
+  if (k == nd)
+    s = nd == 2 ? std::sqrt(volume * npc / (double)n_b)
+                : std::cbrt(volume * npc / (double)n_b);
+  else if (k > 0)
+    s = std::pow(volume * npc / (double)n_b, 1.0 / k);
+  // No axis may need more than 65535 cells.
+  if (!(s > 0) || !std::isfinite(s)) s = max_ext > 0 ? max_ext : 1.0;
+  s = std::max(s, max_ext / 65535.0);
   // Keep the dense table within budget: at most max(2^20, 4 |B|) cells and
   // per-axis sizes that fit the 32-bit cell id.
   const double cap = std::max<double>(1u << 20, 4.0 * (double)n_b);

@@ -15,6 +15,9 @@
 #pragma clang fp contract(off)
 
 #define HD_DEV __device__ __forceinline__
+// Also callable from host code (the CPU early break visits A and B in the
+// same pseudo-random order as the device pipelines).
+#define HD_HOST_DEV __host__ __device__ __forceinline__
 
 namespace hd {
 namespace dev {
@@ -93,7 +96,7 @@ HD_DEV bool WaveAny(bool c) { return __ballot(c) != 0ull; }
 // Feistel permutation of [0, n) (struct Feistel in hd_internal.h): a 4-round
 // Feistel network on 2*half bits (domain < 4n) with cycle walking back into
 // range.
-HD_DEV uint32_t Hash32(uint32_t x) {
+HD_HOST_DEV uint32_t Hash32(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352du;
   x ^= x >> 15;
@@ -102,7 +105,7 @@ HD_DEV uint32_t Hash32(uint32_t x) {
   return x;
 }
 
-HD_DEV uint32_t FeistelOnce(const Feistel& f, uint32_t x) {
+HD_HOST_DEV uint32_t FeistelOnce(const Feistel& f, uint32_t x) {
   uint32_t l = x >> f.half, r = x & f.mask;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -113,7 +116,7 @@ HD_DEV uint32_t FeistelOnce(const Feistel& f, uint32_t x) {
   return (l << f.half) | r;
 }
 
-HD_DEV uint32_t Permute(const Feistel& f, uint32_t i) {
+HD_HOST_DEV uint32_t Permute(const Feistel& f, uint32_t i) {
   uint32_t y = FeistelOnce(f, i);
   while (y >= f.n) y = FeistelOnce(f, y);
   return y;

@@ -270,8 +270,8 @@ int main(int argc, char** argv) {
   if (argc == 1 || !Parse(argc, argv, &f)) {
     fprintf(stderr,
             "Usage: hd_exec -input1 A -input2 B -input_type wkt|off|ply|bin|synthetic "
-            "-n_dims 2|3 [-variant eb|grid|rt|lbvh|hybrid|nn|rt-hdist] [-execution gpu] "
-            "[-repeat 5] [-seed 0] [-check] [-symmetric] [-learned_order] "
+            "-n_dims 2|3 [-variant eb|grid|rt|lbvh|hybrid|nn|rt-hdist] [-execution gpu|cpu] "
+            "[-repeat 5] [-seed 0] [-parallelism -1] [-check|-nocheck] [-symmetric] [-learned_order] "
             "[-n_points_cell 8] [-max_hit 256] [-auto_tune] [-limit N] "
             "[-stats_n_points_cell 10] [-vary_params -sample_rate_list r,.. "
             "-eb_only_threshold_list n,.. -max_hit_list n,.. -n_points_cell_list n,.. "
@@ -315,7 +315,9 @@ int main(int argc, char** argv) {
   cfg.eb_only_threshold = (uint32_t)f.Int("eb_only_threshold", 0);
   cfg.auto_tune = f.Bool("auto_tune", false) ? 1 : 0;
   cfg.stats_n_points_cell = (int32_t)f.Int("stats_n_points_cell", 10);
-  if (f.Bool("check", false)) cfg.flags |= HD_FLAG_CHECK;
+  // -check defaults to on, as the reference's flag (src/flags.cc:49): every
+  // run is compared with the host early break (-nocheck turns it off).
+  if (f.Bool("check", true)) cfg.flags |= HD_FLAG_CHECK;
   if (f.Bool("symmetric", false)) cfg.flags |= HD_FLAG_SYMMETRIC;
   if (f.Bool("learned_order", false)) cfg.flags |= HD_FLAG_LEARNED_ORDER;
   if (!f.Bool("shuffle", true)) cfg.flags |= HD_FLAG_NO_SHUFFLE;

@@ -237,6 +237,21 @@ int EbFinishList(hd_context* ctx, int n_dims, const float* a, uint32_t n_a,
                  uint32_t n_list, unsigned long long* key,
                  unsigned long long* ev, EbRoundStats* rs);
 
+// Host early break (cpu_eb.cpp): the reference's CPU path on host AoS
+// inputs with `parallelism` threads (<= 0: hardware_concurrency).
+struct CpuRunInfo {
+  int threads = 0;
+  double shuffle_ms = 0;  // gather into visit order
+  double compute_ms = 0;  // the scan (ReportedTime)
+  const char* simd = "";
+};
+int HostThreads(int parallelism);
+int CpuEarlyBreak(int n_dims, const float* a, uint64_t n_a, const float* b,
+                  uint64_t n_b, uint32_t flags, int32_t seed, int parallelism,
+                  hd_directed_result* out, CpuRunInfo* info);
+// Raw MBR (lower[nd], upper[nd]) of host AoS points.
+void HostPointsMbr(int n_dims, const float* pts, uint64_t n, float* mbr);
+
 // Exact NN of queries against ref by index kind.
 int NearestNeighbors(hd_context* ctx, int index, int n_dims, const float* q,
                      uint64_t n_q, const float* ref, uint64_t n_ref,

@@ -24,6 +24,8 @@ int ExclusiveScanU32(hd_context* ctx, const uint32_t* in, uint32_t* out,
 
 // Host construction of a Feistel permutation of [0, n) (hd_device.h).
 Feistel MakeFeistel(uint32_t n, uint64_t seed, uint32_t stream_id);
+// The same permutation evaluated on the host: position i -> index.
+uint32_t HostPermute(const Feistel& f, uint32_t i);
 
 // p[i] = i for i < n.
 int IotaU32(hd_context* ctx, uint32_t* p, uint32_t n);

@@ -505,7 +505,9 @@ int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
   f.qmax = (float)levels;
   for (int d = 0; d < nd; ++d) {
     f.lo[d] = mbr[d];
-    f.scale[d] = (float)(levels / ((double)mbr[nd + d] - (double)mbr[d]));
+    // A zero extent (a flat set whose widening rounded away) quantises to 0.
+    const double ext = (double)mbr[nd + d] - (double)mbr[d];
+    f.scale[d] = ext > 0 ? (float)(levels / ext) : 0.f;
   }
   uint32_t *keys = nullptr, *keys_s = nullptr;
   uint32_t *vals = nullptr, *vals_s = nullptr;

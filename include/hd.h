@@ -60,20 +60,34 @@ extern "C" {
  * three cell diagonals beyond its first hit). */
 #define HD_VARIANT_RT_HDIST 5
 
-/* Execution (src/run_config.h:18). This library serves GPU execution only;
- * HD_EXEC_CPU returns HD_EUNSUPPORTED. */
+/* Execution (src/run_config.h:18). CPU runs the early-break variant on the
+ * host with `parallelism` threads (hd_directed_host); other variants return
+ * HD_EUNSUPPORTED on the CPU, as the reference's base class aborts with
+ * "CalculateDistance on the CPU is not implemented"
+ * (src/hausdorff_distance.h:21-26). CPU execution needs no GPU. */
 #define HD_EXEC_CPU 0
 #define HD_EXEC_GPU 1
 
 /* hd_run_config.flags */
 #define HD_FLAG_SYMMETRIC 0x1u       /* H = max(h(A->B), h(B->A)) */
-#define HD_FLAG_CHECK 0x2u           /* re-run with the plain EB variant and
-                                        compare bits (cf. -check, :366-388) */
+#define HD_FLAG_CHECK 0x2u           /* re-run the host early break with
+                                        hardware_concurrency threads and
+                                        compare cmax2 bits and arg-max
+                                        (-check, run_hausdorff_distance.cu:
+                                        366-388) */
 #define HD_FLAG_LEARNED_ORDER 0x4u   /* visit A in descending MLP-predicted
                                         NN distance */
 #define HD_FLAG_COUNT_EVALS 0x8u     /* count d2 evaluations */
 #define HD_FLAG_INPUT_ON_DEVICE 0x10u/* a, b are device pointers */
 #define HD_FLAG_NO_SHUFFLE 0x20u     /* visit A and B in input order */
+#define HD_FLAG_SINGLE_PASS 0x40u    /* early break as one pass over A in the
+                                        visit order: every point scans B
+                                        until it breaks, cmax2 starts at 0
+                                        (the reference's GPU kernel,
+                                        early_break.h:117-219); no seed,
+                                        rounds or matrix-core filter. EB
+                                        variant only: a comparison
+                                        baseline, same result */
 
 /* Reference RunConfig (src/run_config.h:20-61) restricted to the hot path,
  * with the defaults of src/flags.cc:3-57 (hd_run_config_init). */
@@ -165,12 +179,29 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
  * overload CalculateDistance(stream, d_a, d_b) of the reference
  * (src/hausdorff_distance.h:27-32), without its in-place shuffle
  * (early_break.h:132-136): d_a and d_b are read only. `variant` is
- * HD_VARIANT_*, `flags` takes LEARNED_ORDER, COUNT_EVALS, NO_SHUFFLE.
+ * HD_VARIANT_*, `flags` takes LEARNED_ORDER, COUNT_EVALS, NO_SHUFFLE,
+ * SINGLE_PASS.
  * `model` (may be NULL) is used with LEARNED_ORDER; NULL trains one. */
 int hd_directed_device(hd_context* ctx, int variant, int n_dims,
                        const float* d_a, uint64_t n_a, const float* d_b,
                        uint64_t n_b, uint32_t flags, int32_t seed,
                        const hd_mlp* model, hd_directed_result* out);
+
+/* One directed h(A->B) on the host CPU: the reference's CPU overload
+ * CalculateDistance(std::vector&, std::vector&)
+ * (src/hd_impl/hausdorff_distance_early_break.h:44-115), without its
+ * in-place shuffle: a and b are host AoS arrays and are read only. A and B
+ * are visited in the library's pseudo-random order (NO_SHUFFLE: input
+ * order), A is split statically over `parallelism` threads (<= 0:
+ * std::thread::hardware_concurrency(), run_hausdorff_distance.cu:222-224),
+ * and every point stops scanning once its running minimum is strictly below
+ * the shared cmax2. out->time_ms is the scan time without the gather
+ * (ReportedTime, :101-112); out->evals counts evaluated pairs per visited
+ * block of 32. Same cmax2 bits, arg-max and NN as every GPU variant.
+ * `flags` takes NO_SHUFFLE; LEARNED_ORDER returns HD_EUNSUPPORTED. */
+int hd_directed_host(int n_dims, const float* a, uint64_t n_a, const float* b,
+                     uint64_t n_b, uint32_t flags, int32_t seed,
+                     int32_t parallelism, hd_directed_result* out);
 
 /* Exact nearest neighbour of every query in `ref` on the device: the
  * per-point minimum the EB loop computes when it does not break. index:

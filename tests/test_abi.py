@@ -73,7 +73,9 @@ PTS = np.zeros((4, 2), np.float32)
     ({"n_dims": 1}, _lib.HD_EINVAL, "dimensions"),
     ({"variant": 9}, _lib.HD_EINVAL, "variant"),
     ({"execution": 5}, _lib.HD_EINVAL, "execution"),
-    ({"execution": _lib.HD_EXEC_CPU}, _lib.HD_EUNSUPPORTED, "CPU"),
+    # the CPU path serves the early break only (hausdorff_distance.h:21-26)
+    ({"execution": _lib.HD_EXEC_CPU, "variant": _lib.HD_VARIANT_HYBRID},
+     _lib.HD_EUNSUPPORTED, "CPU"),
     ({"repeat": 0}, _lib.HD_EINVAL, "repeat"),
     ({"abi_version": 99}, _lib.HD_EINVAL, "abi_version"),
 ])

@@ -44,3 +44,55 @@ def test_batch_ragged_3d_and_tiny_pairs(ctx):
     a = rng.random((oa[-1], 3)).astype(np.float32)
     b = rng.random((ob[-1], 3)).astype(np.float32)
     _check(ctx, a, oa, b, ob, 3)
+
+
+def test_batch_offsets_rewritten_in_place(ctx):
+    """The same device tensors holding another ragged split: the cached plan
+    must not be reused (its pair boundaries would be stale)."""
+    import torch
+
+    rng = np.random.default_rng(11)
+    a = rng.random((30_000, 2)).astype(np.float32)
+    b = rng.random((24_000, 2)).astype(np.float32)
+    da = torch.from_numpy(a).cuda()
+    db = torch.from_numpy(b).cuda()
+
+    def split(n_pts, n_pairs, seed):
+        cuts = np.sort(np.random.default_rng(seed).choice(np.arange(1, n_pts), n_pairs - 1,
+                                                          replace=False))
+        return np.concatenate([[0], cuts, [n_pts]]).astype(np.int64)
+
+    oa1, ob1 = split(len(a), 40, 1), split(len(b), 40, 2)
+    oa2, ob2 = split(len(a) - 500, 40, 3), split(len(b) - 7, 40, 4)
+    doa = torch.from_numpy(oa1).cuda()
+    dob = torch.from_numpy(ob1).cuda()
+    for oa, ob in ((oa1, ob1), (oa2, ob2), (oa1, ob1)):
+        doa.copy_(torch.from_numpy(oa))
+        dob.copy_(torch.from_numpy(ob))
+        torch.cuda.synchronize()
+        res = hd.batch_directed(ctx, da, doa, db, dob)
+        for p in range(40):
+            o = oracle.eb_directed(a[oa[p]:oa[p + 1]], b[ob[p]:ob[p + 1]], n_threads=4)
+            assert (res[p].cmax2_bits, res[p].argmax, res[p].nn) == (
+                o.cmax2_bits, o.argmax, o.nn), p
+
+
+def test_batch_offsets_are_validated(ctx):
+    import torch
+
+    a = torch.rand((100, 2), device="cuda")
+    b = torch.rand((80, 2), device="cuda")
+    good_a = torch.tensor([0, 50, 100], dtype=torch.int64, device="cuda")
+    good_b = torch.tensor([0, 40, 80], dtype=torch.int64, device="cuda")
+    with pytest.raises(hd.HDError):  # past the end of B
+        hd.batch_directed(ctx, a, good_a, b, torch.tensor([0, 40, 81], dtype=torch.int64,
+                                                           device="cuda"))
+    with pytest.raises(hd.HDError):  # wrong dtype
+        hd.batch_directed(ctx, a, good_a.to(torch.int32), b, good_b)
+    with pytest.raises(hd.HDError):  # host offsets
+        hd.batch_directed(ctx, a, good_a.cpu(), b, good_b)
+    with pytest.raises(hd.HDError):  # decreasing offsets (checked by the library)
+        hd.batch_directed(ctx, a, torch.tensor([0, 60, 50], dtype=torch.int64, device="cuda"),
+                          b, good_b)
+    res = hd.batch_directed(ctx, a, good_a, b, good_b)
+    assert len(res) == 2

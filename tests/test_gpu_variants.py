@@ -202,3 +202,22 @@ def test_hybrid_sampled_eb_only_decision(ctx):
     # point, so every count is 1.
     one = np.repeat(np.array([[0.5, 0.5]], np.float32), 3000, axis=0)
     assert set(sample_counts(ctx, a, one, rate=0.05).tolist()) == {1}
+
+
+@pytest.mark.parametrize("variant", [hd.Variant.GRID, hd.Variant.LBVH, hd.Variant.HYBRID,
+                                     hd.Variant.EB], ids=lambda v: v.name)
+def test_flat_set_at_large_coordinates(ctx, variant):
+    """B lies on the plane z = 1000 (3D) or the line y = 1000 (2D): the +1e-5
+    widening of its zero extent rounds away in FP32, so the grid must size
+    itself from the other axes instead of from a zero volume."""
+    rng = np.random.default_rng(21)
+    for nd in (3, 2):
+        b = rng.random((200_000, nd)).astype(np.float32)
+        b[:, nd - 1] = np.float32(1000.0)
+        a = rng.random((50_000, nd)).astype(np.float32)
+        a[:, nd - 1] = (np.float32(1000.0) + rng.normal(0, 0.05, 50_000)).astype(np.float32)
+        print(f"[flat] {variant.name} {nd}D", flush=True)
+        r = run(a, b, variant, ctx=ctx)
+        o_ab, o_ba = oracle.eb_symmetric(a, b)
+        assert triple(r.ab) == triple(o_ab)
+        assert triple(r.ba) == triple(o_ba)
