@@ -75,12 +75,16 @@ def stage_times(lib, ctx, reset: bool) -> dict:
     return json.loads(buf.value.decode())
 
 
-def time_steps(hd, ctx, variant, da, db, steps, warmup, learned, model=None):
+def time_steps(hd, ctx, variant, da, db, steps, warmup, learned, model=None,
+               single_pass=False):
+    """W untimed warm-up steps, then K timed steps bracketed by device
+    synchronisation; returns (ms per step over the K, results, stage times)."""
     import torch
 
     lib = hd.load()
+    kw = dict(seed=0, learned_order=learned, model=model, single_pass=single_pass)
     for _ in range(warmup):
-        hd.directed_device(ctx, variant, da, db, seed=0, learned_order=learned, model=model)
+        hd.directed_device(ctx, variant, da, db, **kw)
     torch.cuda.synchronize()
     lib.hd_context_set_profiling(ctx.handle, 1)
     stage_times(lib, ctx, reset=True)
@@ -88,13 +92,19 @@ def time_steps(hd, ctx, variant, da, db, steps, warmup, learned, model=None):
     t0 = time.perf_counter()
     results = []
     for _ in range(steps):
-        results.append(hd.directed_device(ctx, variant, da, db, seed=0, learned_order=learned,
-                                          model=model))
+        results.append(hd.directed_device(ctx, variant, da, db, **kw))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     stages = stage_times(lib, ctx, reset=True)
     lib.hd_context_set_profiling(ctx.handle, 0)
     return (t1 - t0) * 1e3 / steps, results, stages
+
+
+def spread(results) -> dict:
+    """Median, minimum and maximum of the per-step device times (HIP events
+    around each call)."""
+    t = sorted(r.time_ms for r in results)
+    return {"median_ms": t[len(t) // 2], "min_ms": t[0], "max_ms": t[-1], "n": len(t)}
 
 
 def next_rows(hd, ctx, da, db, n_a: int, n_b: int) -> dict:
@@ -165,29 +175,28 @@ def next_rows(hd, ctx, da, db, n_a: int, n_b: int) -> dict:
 
 
 def eb_roofline(stages: dict, steps: int) -> dict:
-    """Roofline of the dominant early-break kernel over the timed plain-order
-    steps: the kernel whose stages take the most device time (HIP events on
-    the library's stream around each stage), its average launch duration,
-    and its algorithmic work per launch."""
+    """Roofline of the dominant early-break kernel over the timed steps: the
+    kernel whose stages take the most device time (HIP events on the
+    library's stream around each stage), its average dispatch duration
+    (stage time over the kernel dispatches the stages made), and its
+    algorithmic work per dispatch (the kernels' own evaluation counters).
+    `traffic` is null here: HBM bytes per dispatch come from rocprofv3
+    counter runs of the same command (FETCH_SIZE / WRITE_SIZE), kept with
+    the command in profiles/."""
     per = {}
     for name, k in EB_KERNELS.items():
         ms = sum(stages.get(st, {}).get("ms", 0.0) for st in k["stages"])
-        calls = sum(stages.get(st, {}).get("calls", 0) for st in k["stages"])
+        disp = sum(stages.get(st, {}).get("dispatches", 0) for st in k["stages"])
         evals = sum(stages.get(st, {}).get("evals", 0) for st in k["stages"])
         points = sum(stages.get(st, {}).get("points", 0) for st in k["stages"])
-        per[name] = (ms, calls, evals, points)
+        per[name] = (ms, disp, evals, points)
     name = max(per, key=lambda n: per[n][0])
-    ms, calls, evals, points = per[name]
-    launches = max(1, calls)
+    ms, disp, evals, points = per[name]
+    launches = max(1, disp)
     t_kernel = ms / launches / 1e3
-    traffic = None
-    traffic_file = ROOT / "profiles" / f"traffic_{name}_gmm_10m_2d.json"
-    if traffic_file.exists():
-        tr = json.loads(traffic_file.read_text())
-        if tr.get("kernel") == EB_KERNELS[name]["full"]:
-            traffic = tr.get("hbm_bytes_per_launch")
-    other = {n: {"ms_per_step": v[0] / max(1, steps), "launches_per_step": v[1] / max(1, steps),
-                 "evals_per_step": v[2] / max(1, steps)} for n, v in per.items()}
+    other = {n: {"ms_per_step": v[0] / max(1, steps), "dispatches_per_step": v[1] / max(1, steps),
+                 "evals_per_step": v[2] / max(1, steps),
+                 "ms_per_dispatch": v[0] / max(1, v[1])} for n, v in per.items()}
     if name.startswith("EbMfmaKernel"):
         flop = MFMA_FLOP_PER_PAIR * evals / launches
         achieved = flop / t_kernel / 1e12 if t_kernel > 0 else None
@@ -195,14 +204,12 @@ def eb_roofline(stages: dict, steps: int) -> dict:
             "bound": "mfma", "kernel": EB_KERNELS[name]["full"], "achieved": achieved,
             "peak": MFMA_F32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / MFMA_F32_TFLOPS if achieved else None,
-            "traffic": traffic, "algorithmic_flop": flop, "kernel_ms": t_kernel * 1e3,
-            "launches_per_step": calls / max(1, steps),
-            # the other two views of the same launches: the streaming model
-            # (8 B of B per evaluated pair, as if nothing were reused) and the
-            # measured HBM rate (calibrated counters over the launch time)
+            "traffic": None, "algorithmic_flop": flop, "kernel_ms": t_kernel * 1e3,
+            "dispatches_per_step": disp / max(1, steps),
+            # the streaming view of the same dispatches: 8 B of B per
+            # evaluated pair, as if no tile were reused (above 1: reuse)
             "streaming_model_frac": (8.0 * evals / launches / t_kernel / 1e9 / HBM_PEAK_GBS)
             if t_kernel > 0 else None,
-            "hbm_measured_gbps": (traffic / t_kernel / 1e9) if traffic and t_kernel > 0 else None,
             "kernels": other,
         }
     bytes_alg = 8.0 * (points + evals) / launches
@@ -211,19 +218,74 @@ def eb_roofline(stages: dict, steps: int) -> dict:
         "bound": "hbm", "kernel": EB_KERNELS[name]["full"], "achieved": achieved,
         "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS if achieved else None,
-        "traffic": traffic, "algorithmic_bytes": bytes_alg, "kernel_ms": t_kernel * 1e3,
-        "launches_per_step": calls / max(1, steps),
+        "traffic": None, "algorithmic_bytes": bytes_alg, "kernel_ms": t_kernel * 1e3,
+        "dispatches_per_step": disp / max(1, steps),
         "valu_frac": (evals / launches * LANE_OPS_PER_EVAL[2] / t_kernel / VALU_LANE_OPS)
         if t_kernel > 0 else None,
         "kernels": other,
     }
 
 
+# Index queries: algorithmic bytes per dispatch = P per query point + the
+# index records read (8 B per grid cell lookup: start and end; 64 B per LBVH
+# node record: both child boxes and ids) + P per candidate B point.
+INDEX_KERNELS = {
+    "grid": {"kernel": "GridQueryKernel<2>", "stages": ("grid_seed", "grid_query"),
+             "record_bytes": 8},
+    "lbvh": {"kernel": "LbvhQueryKernel<2>", "stages": ("lbvh_seed", "lbvh_query"),
+             "record_bytes": 64},
+}
+
+
+def index_roofline(variant: str, stages: dict, n_dims: int = 2) -> dict | None:
+    k = INDEX_KERNELS.get(variant)
+    if k is None:
+        return None
+    p = 4 * n_dims
+    ms = sum(stages.get(st, {}).get("ms", 0.0) for st in k["stages"])
+    disp = sum(stages.get(st, {}).get("dispatches", 0) for st in k["stages"])
+    evals = sum(stages.get(st, {}).get("evals", 0) for st in k["stages"])
+    points = sum(stages.get(st, {}).get("points", 0) for st in k["stages"])
+    lookups = sum(stages.get(st, {}).get("lookups", 0) for st in k["stages"])
+    if disp == 0 or ms <= 0:
+        return None
+    bytes_alg = (p * points + k["record_bytes"] * lookups + p * evals) / disp
+    t = ms / disp / 1e3
+    gbs = bytes_alg / t / 1e9
+    return {"bound": "hbm", "kernel": k["kernel"], "achieved": gbs, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": bytes_alg,
+            "kernel_ms": t * 1e3, "dispatches": disp,
+            "evals_per_point": evals / max(1, points),
+            "records_per_point": lookups / max(1, points)}
+
+
+# The host-CPU share of one GPU on the benchmark machines: worker pools are
+# sized to it, whatever the machine reports as hardware concurrency.
+CPU_SHARE = 16
+
+
+def cpu_info() -> dict:
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "hardware_concurrency": os.cpu_count(), "affinity": aff,
+            "threads_used": max(1, min(aff or 1, CPU_SHARE))}
+
+
 def cpu_baseline(a: np.ndarray, b: np.ndarray, target_s: float = 12.0) -> dict:
-    """CPU oracle early break on a random sample of A against all of B."""
+    """CPU oracle early break on a random sample of A against all of B, on
+    the host threads this process may use (at most the GPU's CPU share)."""
     from oracle import oracle
 
-    cores = min(os.cpu_count() or 1, 16)
+    info = cpu_info()
+    cores = info["threads_used"]
     rng = np.random.default_rng(0)
     k = min(100_000, a.shape[0])
     idx = rng.choice(a.shape[0], k, replace=False)
@@ -241,8 +303,23 @@ def cpu_baseline(a: np.ndarray, b: np.ndarray, target_s: float = 12.0) -> dict:
         "cores": int(r.n_threads),
         "kind": "port",
         "sample": f"{k} random A points (seed 0) vs all {b.shape[0]} B points, "
-                  f"directed A->B, {r.compared_pairs} d2 evaluations in {t:.2f} s",
+                  f"directed A->B, {r.compared_pairs} d2 evaluations in {t:.2f} s; "
+                  f"host {info['model']}, {info['hardware_concurrency']} hardware threads, "
+                  f"{info['affinity']} in this process's affinity, {cores} used "
+                  f"(the GPU's host-CPU share is {CPU_SHARE})",
     }
+
+
+def host_early_break(hd, a: np.ndarray, b: np.ndarray, want: tuple) -> dict:
+    """The library's own CPU execution path (hd_directed_host: the early
+    break on the host, AVX2) on the whole workload, same bits required."""
+    info = cpu_info()
+    r = hd.directed_host(a, b, parallelism=info["threads_used"])
+    if (r.cmax2_bits, r.argmax, r.nn) != want:
+        raise RuntimeError("host early break differs from the GPU result")
+    return {"a_points_per_s": a.shape[0] / (r.time_ms / 1e3), "ms": r.time_ms,
+            "threads": info["threads_used"], "evals": r.evals,
+            "note": "execution = CPU of the library (not the oracle), whole workload"}
 
 
 def copy_probe(mib: int) -> None:
@@ -418,8 +495,18 @@ def main() -> None:
     log(f"plain: {ms_plain:.3f} ms/step, cmax2 bits {r0.cmax2_bits} argmax {r0.argmax} "
         f"evals {r0.evals}")
     log(f"plain stages: {json.dumps(st_plain)}")
+    # The reference GPU kernel's schedule (early_break.h:117-219): one pass
+    # over the pseudo-random order, cmax2 from 0, no ranking or filter.
+    log("plain order, single pass")
+    ms_single, res_single, _ = time_steps(hd, ctx, hd.Variant.EB, da, db, args.steps,
+                                          args.warmup, learned=False, single_pass=True)
+    rs = res_single[-1]
+    if (rs.cmax2_bits, rs.argmax, rs.nn) != (r0.cmax2_bits, r0.argmax, r0.nn):
+        raise RuntimeError("single pass differs from the pipeline")
+    log(f"single pass: {ms_single:.3f} ms/step, evals {rs.evals}")
     ms_learned = None
     st_learned = None
+    res_learned = None
     r1 = None
     train_s = None
     model = None
@@ -465,7 +552,8 @@ def main() -> None:
             raise RuntimeError(f"{name} result differs from early break")
         log(f"{name}: {ms_v:.3f} ms/step, evals {rv.evals}, stages {json.dumps(st_v)}")
         other[name] = {"ms_per_step": ms_v, "a_points_per_s": a.shape[0] / (ms_v / 1e3),
-                       "evals": rv.evals,
+                       "evals": rv.evals, "spread": spread(res_v),
+                       "query_roofline": index_roofline(name, st_v),
                        "stages_ms": {k: x["ms"] / max(1, x["calls"]) for k, x in st_v.items()}}
 
     # Symmetric H = max(h(A->B), h(B->A)) with the hybrid selector.
@@ -504,10 +592,14 @@ def main() -> None:
     roofline = eb_roofline(st_plain, args.steps)
 
     cpu = None
+    host_eb = None
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline(a, b)
         log(f"cpu: {cpu['value']:.4g} A-points/s on {cpu['cores']} threads")
+        log("host early break (execution = CPU)")
+        host_eb = host_early_break(hd, a, b, (r0.cmax2_bits, r0.argmax, r0.nn))
+        log(f"host early break: {host_eb['ms']:.1f} ms")
 
     line = {
         "metric": "A-points/sec, 10M x 10M 2D HD (directed, early break), learned-order vs plain",
@@ -529,8 +621,24 @@ def main() -> None:
             "order": "learned" if use_learned else "plain",
             "plain_ms_per_step": ms_plain,
             "plain_value": n_a / (ms_plain / 1e3),
+            "plain_spread": spread(res_plain),
+            # the pipeline without the network: seed, a ranking round on a
+            # sample of the pseudo-random order, top completions, budgeted
+            # rounds on the matrix-core filter, segmented tail
+            "plain_is": "ranked pipeline (pseudo-random order, no network)",
+            "single_pass_ms_per_step": ms_single,
+            "single_pass_value": n_a / (ms_single / 1e3),
+            "single_pass_spread": spread(res_single),
+            "single_pass_is": "one pass over the pseudo-random order, cmax2 from 0, exact "
+                              "scan only: the reference GPU kernel's schedule",
+            "evals_single_pass": rs.evals,
             "learned_ms_per_step": ms_learned,
+            "learned_spread": spread(res_learned) if use_learned else None,
             "learned_speedup": (ms_plain / ms_learned) if use_learned else None,
+            "learned_speedup_vs_single_pass": (ms_single / ms_learned) if use_learned else None,
+            "learned_speedup_median": (spread(res_plain)["median_ms"]
+                                       / spread(res_learned)["median_ms"])
+            if use_learned else None,
             "model_train_s": train_s,
             "model_final_loss": final_loss,
             "cmax2_bits": r0.cmax2_bits,
@@ -543,6 +651,7 @@ def main() -> None:
             "other_variants": other,
             "symmetric_hybrid": sym,
             "next_rows": extra,
+            "host_early_break": host_eb,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

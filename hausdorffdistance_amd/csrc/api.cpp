@@ -348,22 +348,25 @@ int StageBegin(hd_context* ctx, const char* name) {
     return -1;
   }
   ctx->pending_stages.push_back(st);
-  return (int)ctx->pending_stages.size() - 1;
+  ctx->open_stage = (int)ctx->pending_stages.size() - 1;
+  return ctx->open_stage;
 }
 
 void StageEnd(hd_context* ctx, int handle) {
   if (handle < 0 || handle >= (int)ctx->pending_stages.size()) return;
+  if (ctx->open_stage == handle) ctx->open_stage = -1;
   if (hipEventRecord(ctx->pending_stages[handle].end, ctx->stream) !=
       hipSuccess)
     (void)hipGetLastError();
 }
 
 void StageCount(hd_context* ctx, const char* name, uint64_t evals,
-                uint64_t points) {
+                uint64_t points, uint64_t lookups) {
   if (!ctx->profiling) return;
   StageStat& s = ctx->stage_stats[name];
   s.evals += evals;
   s.points += points;
+  s.lookups += lookups;
 }
 
 int ResolveStages(hd_context* ctx) {
@@ -375,6 +378,7 @@ int ResolveStages(hd_context* ctx) {
       StageStat& s = ctx->stage_stats[st.name];
       s.total_ms += ms;
       s.calls += 1;
+      s.dispatches += st.dispatches;
     } else {
       (void)hipGetLastError();
     }
@@ -382,6 +386,7 @@ int ResolveStages(hd_context* ctx) {
     ctx->event_pool.push_back(st.end);
   }
   ctx->pending_stages.clear();
+  ctx->open_stage = -1;
   return HD_OK;
 }
 
@@ -502,8 +507,10 @@ int hd_context_stage_times(hd_context* ctx, char* json, uint64_t cap,
     os << (first ? "" : ", ") << "\"" << JsonEscape(kv.first)
        << "\": {\"ms\": " << Num(kv.second.total_ms)
        << ", \"calls\": " << kv.second.calls
+       << ", \"dispatches\": " << kv.second.dispatches
        << ", \"evals\": " << kv.second.evals
-       << ", \"points\": " << kv.second.points << "}";
+       << ", \"points\": " << kv.second.points
+       << ", \"lookups\": " << kv.second.lookups << "}";
     first = false;
   }
   os << "}";

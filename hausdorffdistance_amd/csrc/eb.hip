@@ -959,6 +959,7 @@ int LaunchScan(hd_context* ctx, ScanArgs& args, uint32_t pos_begin,
   const uint32_t per_seg = std::max(1u, n_waves / args.n_segs);
   args.claim = std::min<uint32_t>(
       kWaveSize, std::max<uint32_t>(1u, (n_entries + per_seg - 1) / per_seg));
+  StageDispatch(ctx);
   EbScanKernel<ND, kP, kS><<<blocks, kBlock, 0, ctx->stream>>>(args);
   HD_HIP_TRY(hipGetLastError());
   return HD_OK;
@@ -1204,6 +1205,7 @@ int LaunchMfma(hd_context* ctx, const float* a, int order_mode,
     args.claim = std::min<uint32_t>(
         kWaveSize, std::max<uint32_t>(16u, (n_entries + n_waves - 1) / n_waves));
   }
+  StageDispatch(ctx);
   if (grouped)
     EbMfmaKernel<ND, true><<<blocks, kBlock, 0, ctx->stream>>>(args);
   else
@@ -1273,6 +1275,7 @@ int LaunchMfmaSplit(hd_context* ctx, const float* a, const uint32_t* list,
   const uint32_t per_seg = std::max(1u, n_waves / n_segs);
   args.claim = std::min<uint32_t>(
       kWaveSize, std::max<uint32_t>(16u, (n_list + per_seg - 1) / per_seg));
+  StageDispatch(ctx);
   EbMfmaKernel<ND, false><<<blocks, kBlock, 0, ctx->stream>>>(args);
   HD_HIP_TRY(hipGetLastError());
   return HD_OK;

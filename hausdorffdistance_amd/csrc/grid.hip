@@ -180,6 +180,7 @@ struct QueryState {
   uint32_t best_j;
   uint32_t evals;
   uint32_t cells;
+  uint32_t lookups;  // cell table reads
   bool stop;     // early break or cell budget exhausted
   bool broke;    // best < cmax2
   bool over;     // cell budget exhausted
@@ -201,6 +202,7 @@ HD_DEV void ScanCell(const GridDev& g, const float* qf, const double* qd,
     return;
   }
   const uint32_t lin = Linear<ND>(g, c);
+  st.lookups++;
   const uint32_t b0 = g.start[lin], b1 = g.end[lin];
   for (uint32_t k = b0; k < b1; ++k) {
     float d2;
@@ -298,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void GridQueryKernel(GridQueryArgs args) {
   const uint32_t lane = LaneId();
   const uint32_t lanes_total = gridDim.x * blockDim.x;
   const uint32_t wave_base = (blockIdx.x * blockDim.x + threadIdx.x) - lane;
-  unsigned long long evals = 0;
+  unsigned long long evals = 0, lookups = 0;
   for (uint32_t base = args.pos_begin + wave_base; base < args.pos_end;
        base += lanes_total) {
     const uint32_t pos = args.stride > 1u ? (base + lane) * args.stride : base + lane;
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void GridQueryKernel(GridQueryArgs args) {
                         !(args.skip_stride > 1u && pos % args.skip_stride == 0u &&
                           pos / args.skip_stride < args.skip_n);
     const float cmax2 = args.eb ? LoadCmax2(args.key) : 0.f;
-    QueryState st{kFltMax, 0xFFFFFFFFu, 0, 0, false, false, false};
+    QueryState st{kFltMax, 0xFFFFFFFFu, 0, 0, 0, false, false, false};
     uint32_t idx = 0;
     if (active) {
       idx = args.order_mode == 1 ? Permute(args.perm, pos)
@@ -333,6 +335,7 @@ __global__ __launch_bounds__(kBlock) void GridQueryKernel(GridQueryArgs args) {
       }
     }
     evals += st.evals;
+    lookups += st.lookups;
     if (active && args.nn_d2 != nullptr) {
       args.nn_d2[idx] = st.best;
       args.nn_idx[idx] = st.best_j;
@@ -359,11 +362,15 @@ __global__ __launch_bounds__(kBlock) void GridQueryKernel(GridQueryArgs args) {
       if (lane == 0 && kmax != 0ull) atomicMax(args.key, kmax);
     }
   }
-  // Per-wave evaluation count.
-  unsigned long long e = evals;
+  // Per-wave evaluation and lookup counts.
+  unsigned long long e = evals, l = lookups;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off, 64);
+    l += __shfl_xor(l, off, 64);
+  }
   if (lane == 0 && args.evals != nullptr) atomicAdd(args.evals, e);
+  if (lane == 0 && args.lookups != nullptr) atomicAdd(args.lookups, l);
 }
 
 }  // namespace
@@ -489,6 +496,7 @@ int LaunchGridQuery(hd_context* ctx, const GridQueryArgs& args) {
   int blocks = PersistentBlocks(ctx, kBlock, 8);
   blocks = (int)std::min<uint32_t>((uint32_t)blocks, (n + kBlock - 1) / kBlock);
   blocks = std::max(blocks, 1);
+  StageDispatch(ctx);
   if (args.grid_nd == 2)
     GridQueryKernel<2><<<blocks, kBlock, 0, ctx->stream>>>(args);
   else
@@ -513,10 +521,13 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
 
   // key, seed and query counters, then the early-break counters of the
   // escalated queries.
+  // key, evaluation counters (seed, query, EB finish), then the cell
+  // lookups of the seed and of the query.
   constexpr int kKeys = 3 + kEbCounters;
+  constexpr int kLookups = 2;
   unsigned long long* key = nullptr;
-  HD_TRY(ContextArray(ctx, "grid_key", kKeys, &key));
-  HD_TRY(FillU64(ctx, key, 0ull, kKeys));
+  HD_TRY(ContextArray(ctx, "grid_key", kKeys + kLookups, &key));
+  HD_TRY(FillU64(ctx, key, 0ull, kKeys + kLookups));
   unsigned int* esc_count = nullptr;
   HD_TRY(ContextArray(ctx, "grid_esc_count", 1, &esc_count));
   HD_TRY(FillU32(ctx, esc_count, 0u, 1));
@@ -580,6 +591,7 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
   // waves, so one far query would set its time.
   qa.max_cells = max_cells ? std::max<uint32_t>(max_cells, kSeedCells) : kSeedCells;
   qa.evals = key + 1;
+  qa.lookups = key + kKeys;
   HD_TRY(LaunchGridQuery(ctx, qa));
   StageEnd(ctx, st_seed);
   // Everything else with early break against the running cmax2 (with a
@@ -593,6 +605,7 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
   qa.eb = 1;
   qa.max_cells = max_cells;
   qa.evals = key + 2;
+  qa.lookups = key + kKeys + 1;
   HD_TRY(LaunchGridQuery(ctx, qa));
   StageEnd(ctx, st_q);
   uint32_t n_esc = 0;
@@ -625,11 +638,11 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
     void* pinned = nullptr;
     HD_TRY(ContextPinned(ctx, 256, &pinned));
     unsigned long long* h = static_cast<unsigned long long*>(pinned);
-    HD_HIP_TRY(hipMemcpyAsync(h, key + 1, 8 * (kKeys - 1), hipMemcpyDeviceToHost,
-                              ctx->stream));
+    HD_HIP_TRY(hipMemcpyAsync(h, key + 1, 8 * (kKeys - 1 + kLookups),
+                              hipMemcpyDeviceToHost, ctx->stream));
     HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    StageCount(ctx, "grid_seed", h[0], s0);
-    StageCount(ctx, "grid_query", h[1], n_a - s0);
+    StageCount(ctx, "grid_seed", h[0], s0, h[kKeys - 1]);
+    StageCount(ctx, "grid_query", h[1], n_a - s0, h[kKeys]);
     StageCount(ctx, "grid_build", 0, in.n_b);
     unsigned long long eb_evals = 0;
     for (int i = 2; i < kKeys - 1; ++i) eb_evals += h[i];

@@ -44,6 +44,7 @@ struct GridQueryArgs {
                            // non-empty cells
   unsigned long long* key; // packed max key (may be null)
   unsigned long long* evals;
+  unsigned long long* lookups;  // optional: cell table reads (start, end)
   float* nn_d2;            // optional per-query outputs (indexed by A index)
   uint32_t* nn_idx;
   uint32_t* esc_idx;       // escalations (max_cells exhausted)

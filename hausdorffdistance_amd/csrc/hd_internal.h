@@ -62,12 +62,15 @@ struct DeviceBuffer {
 struct StageStat {
   double total_ms = 0;
   uint64_t calls = 0;
+  uint64_t dispatches = 0;  // kernel dispatches the stage's calls made
   uint64_t evals = 0;
   uint64_t points = 0;
+  uint64_t lookups = 0;  // index records read (grid cells, LBVH nodes)
 };
 struct PendingStage {
   std::string name;
   hipEvent_t begin, end;
+  uint64_t dispatches = 0;
 };
 
 }  // namespace hd
@@ -90,6 +93,7 @@ struct hd_context {
   bool profiling = false;
   std::vector<hipEvent_t> event_pool;
   std::vector<hd::PendingStage> pending_stages;
+  int open_stage = -1;  // pending stage between StageBegin and StageEnd
   std::map<std::string, hd::StageStat> stage_stats;
   // Ordering model trained on demand for HD_FLAG_LEARNED_ORDER without an
   // explicit model, with the inputs it was trained for.
@@ -130,8 +134,15 @@ int ContextActivate(hd_context* ctx);
 int StageBegin(hd_context* ctx, const char* name);
 void StageEnd(hd_context* ctx, int handle);
 void StageCount(hd_context* ctx, const char* name, uint64_t evals,
-                uint64_t points);
+                uint64_t points, uint64_t lookups = 0);
 int ResolveStages(hd_context* ctx);
+// Counts one kernel dispatch against the open stage (profiling only), so
+// per-dispatch times divide by launches actually made.
+inline void StageDispatch(hd_context* ctx) {
+  if (ctx->open_stage >= 0 &&
+      ctx->open_stage < (int)ctx->pending_stages.size())
+    ctx->pending_stages[ctx->open_stage].dispatches += 1;
+}
 
 // Persistent grid size for wave-level kernels of `threads_per_block`.
 inline int PersistentBlocks(const hd_context* ctx, int threads_per_block,

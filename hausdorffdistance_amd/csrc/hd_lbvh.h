@@ -54,6 +54,7 @@ struct LbvhQueryArgs {
   int eb;
   unsigned long long* key;
   unsigned long long* evals;
+  unsigned long long* lookups;  // optional: internal node records read
   float* nn_d2;            // optional per-query outputs (by A index)
   uint32_t* nn_idx;
   uint32_t* esc_idx;       // traversal-stack overflows

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
   const uint32_t lanes_total = gridDim.x * blockDim.x;
   const uint32_t wave_base = blockIdx.x * blockDim.x;
   const bool tie_index = args.nn_d2 != nullptr;
-  unsigned long long evals = 0;
+  unsigned long long evals = 0, nodes = 0;
   for (uint32_t base = args.pos_begin + wave_base; base < args.pos_end;
        base += lanes_total) {
     const uint32_t pos = args.stride > 1u ? (base + lane) * args.stride : base + lane;
@@ -398,6 +398,7 @@ __global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
           continue;
         }
         const LbvhNode rec = t.nodes[node];
+        nodes++;
         const float d0 = BoxD2<ND>(q, rec.lo[0], rec.hi[0]);
         const float d1 = BoxD2<ND>(q, rec.lo[1], rec.hi[1]);
         const bool v0 = tie_index ? d0 <= best : d0 < best;
@@ -447,10 +448,14 @@ __global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
       if (lane == 0 && kmax != 0ull) atomicMax(args.key, kmax);
     }
   }
-  unsigned long long e = evals;
+  unsigned long long e = evals, l = nodes;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off, 64);
+    l += __shfl_xor(l, off, 64);
+  }
   if (lane == 0 && args.evals != nullptr) atomicAdd(args.evals, e);
+  if (lane == 0 && args.lookups != nullptr) atomicAdd(args.lookups, l);
 }
 
 }  // namespace
@@ -594,6 +599,7 @@ int LaunchLbvhQuery(hd_context* ctx, const LbvhQueryArgs& args) {
   int blocks = ctx->cu_count * 12;
   blocks = (int)std::min<uint32_t>((uint32_t)blocks, (n + 63) / 64);
   blocks = std::max(blocks, 1);
+  StageDispatch(ctx);
   if (args.tree_nd == 2)
     LbvhQueryKernel<2><<<blocks, 64, 0, ctx->stream>>>(args);
   else
@@ -651,8 +657,9 @@ int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
   StageEnd(ctx, st_build);
 
   unsigned long long* key = nullptr;
-  HD_TRY(ContextArray(ctx, "lbvh_key", 4, &key));
-  HD_TRY(FillU64(ctx, key, 0ull, 4));
+  // key, evaluations of seed and query, then their node-record reads.
+  HD_TRY(ContextArray(ctx, "lbvh_key", 6, &key));
+  HD_TRY(FillU64(ctx, key, 0ull, 6));
   unsigned int* esc_count = nullptr;
   uint32_t* esc_idx = nullptr;
   float* esc_min = nullptr;
@@ -702,6 +709,7 @@ int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
   qa.pos_end = s0;
   qa.eb = 0;
   qa.evals = key + 1;
+  qa.lookups = key + 4;
   if (morton_order != nullptr) {
     LbvhQueryArgs sa = qa;
     sa.order = morton_order;
@@ -729,6 +737,7 @@ int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
   qa.pos_end = n_a;
   qa.eb = 1;
   qa.evals = key + 2;
+  qa.lookups = key + 5;
   HD_TRY(LaunchLbvhQuery(ctx, qa));
   StageEnd(ctx, st_q);
   // Stack overflows (deep trees) are finished exhaustively.
@@ -746,11 +755,11 @@ int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
   StageEnd(ctx, st_fin);
   if (ctx->profiling) {
     unsigned long long* hh = static_cast<unsigned long long*>(pinned);
-    HD_HIP_TRY(hipMemcpyAsync(hh, key + 1, 16, hipMemcpyDeviceToHost,
+    HD_HIP_TRY(hipMemcpyAsync(hh, key + 1, 40, hipMemcpyDeviceToHost,
                               ctx->stream));
     HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    StageCount(ctx, "lbvh_seed", hh[0], s0);
-    StageCount(ctx, "lbvh_query", hh[1], n_a - s0);
+    StageCount(ctx, "lbvh_seed", hh[0], s0, hh[3]);
+    StageCount(ctx, "lbvh_query", hh[1], n_a - s0, hh[4]);
     StageCount(ctx, "lbvh_overflow", 0, n_over);
   }
   return HD_OK;

@@ -118,6 +118,8 @@ __global__ __launch_bounds__(256) void CellHistogramKernel(
 }
 
 std::string Num(double v) {
+  // JSON has no inf or nan (a flat set's density divides by a zero volume).
+  if (!std::isfinite(v)) return "null";
   std::ostringstream o;
   o.precision(17);
   o << v;
