@@ -125,6 +125,11 @@ SIGNATURES = {
         [_P, C.c_int, C.c_int, _P, C.c_uint64, _P, C.c_uint64, C.c_uint32, C.c_int32, _P,
          C.POINTER(DirectedResultC)],
     ),
+    "hd_directed_device_noalloc": (
+        C.c_int,
+        [_P, C.c_int, _P, C.c_uint64, _P, C.c_uint64, C.c_uint32, C.c_int32,
+         C.POINTER(DirectedResultC)],
+    ),
     "hd_directed_host": (
         C.c_int,
         [C.c_int, _P, C.c_uint64, _P, C.c_uint64, C.c_uint32, C.c_int32, C.c_int32,

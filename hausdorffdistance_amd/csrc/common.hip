@@ -216,10 +216,10 @@ int GatherSoA(hd_context* ctx, int n_dims, const float* aos, uint32_t n,
 }
 
 int HostMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
-            float* mbr) {
+            float* mbr, uint32_t* d_work) {
   if (n == 0) return Fail(HD_EINVAL, "empty point set");
-  uint32_t* d_out = nullptr;
-  HD_TRY(ContextArray(ctx, "mbr_keys", 8, &d_out));
+  uint32_t* d_out = d_work;
+  if (d_out == nullptr) HD_TRY(ContextArray(ctx, "mbr_keys", 8, &d_out));
   // layout: lo[0..nd), hi[nd..2nd), non-finite count at [2nd]
   HD_TRY(FillU32(ctx, d_out, 0xFFFFFFFFu, n_dims));
   HD_TRY(FillU32(ctx, d_out + n_dims, 0u, n_dims + 1));
@@ -240,6 +240,21 @@ int HostMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
   for (int d = 0; d < n_dims; ++d)
     if (mbr[n_dims + d] - mbr[d] == 0.0f)
       mbr[n_dims + d] = (float)((double)mbr[d] + 1e-5);  // mbr.h:288-296
+  return HD_OK;
+}
+
+int LaunchNnIndex(hd_context* ctx, int n_dims, const float* a,
+                  uint32_t argmax, const float* b, uint64_t n_b,
+                  uint32_t cmax2_bits, uint32_t* d_out) {
+  HD_TRY(FillU32(ctx, d_out, 0xFFFFFFFFu, 1));
+  const int grid = GridFor(n_b, 256, ctx->cu_count * 8);
+  if (n_dims == 2)
+    NnIndexKernel<2><<<grid, 256, 0, ctx->stream>>>(a, argmax, b, n_b,
+                                                    cmax2_bits, d_out);
+  else
+    NnIndexKernel<3><<<grid, 256, 0, ctx->stream>>>(a, argmax, b, n_b,
+                                                    cmax2_bits, d_out);
+  HD_HIP_TRY(hipGetLastError());
   return HD_OK;
 }
 
@@ -274,15 +289,8 @@ int FinishDirected(hd_context* ctx, int n_dims, const float* a,
   out->argmax = 0xFFFFFFFFu - (uint32_t)key;
   uint32_t* d_nn = nullptr;
   HD_TRY(ContextArray(ctx, "nn_index", 1, &d_nn));
-  HD_TRY(FillU32(ctx, d_nn, 0xFFFFFFFFu, 1));
-  const int grid = GridFor(n_b, 256, ctx->cu_count * 8);
-  if (n_dims == 2)
-    NnIndexKernel<2><<<grid, 256, 0, ctx->stream>>>(a, out->argmax, b, n_b,
-                                                    out->cmax2_bits, d_nn);
-  else
-    NnIndexKernel<3><<<grid, 256, 0, ctx->stream>>>(a, out->argmax, b, n_b,
-                                                    out->cmax2_bits, d_nn);
-  HD_HIP_TRY(hipGetLastError());
+  HD_TRY(LaunchNnIndex(ctx, n_dims, a, out->argmax, b, n_b, out->cmax2_bits,
+                       d_nn));
   uint32_t* hn = reinterpret_cast<uint32_t*>(&h[12]);
   HD_HIP_TRY(hipMemcpyAsync(hn, d_nn, 4, hipMemcpyDeviceToHost, ctx->stream));
   HD_HIP_TRY(hipStreamSynchronize(ctx->stream));

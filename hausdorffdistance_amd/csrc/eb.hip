@@ -946,7 +946,10 @@ int ResidentBlocks(hd_context* ctx) {
 template <int ND>
 int LaunchScan(hd_context* ctx, ScanArgs& args, uint32_t pos_begin,
                uint32_t n_entries) {
-  HD_TRY(ContextArray(ctx, "eb_cursors", args.n_segs, &args.cursors));
+  // Cursors from the context unless the caller placed them (the
+  // allocation-free path).
+  if (args.cursors == nullptr)
+    HD_TRY(ContextArray(ctx, "eb_cursors", args.n_segs, &args.cursors));
   HD_TRY(FillU32(ctx, args.cursors, pos_begin, args.n_segs));
   const unsigned long long items =
       ((unsigned long long)n_entries + kP - 1) / kP * args.n_segs;
@@ -1655,6 +1658,73 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
 }
 
 }  // namespace
+
+namespace {
+
+template <int ND>
+int EbSinglePassScratchImpl(hd_context* ctx, const float* a, uint32_t n_a,
+                            const float* b, uint32_t n_b, bool shuffle,
+                            int32_t seed, const EbScratch& s,
+                            hd_directed_result* out) {
+  const uint32_t n_bp = (n_b + 63u) & ~63u;
+  if (n_bp > s.max_bp) return Fail(HD_EINVAL, "scratch holds %u points", s.max_bp);
+  // Everything that must start at zero is cleared by fill kernels: the
+  // scratch keeps the previous call's contents.
+  HD_TRY(FillU64(ctx, s.key, 0ull, 2));
+  HD_TRY(FillU32(ctx, s.counters, 0u, 4));
+  const Feistel perm_b = MakeFeistel(n_b, (uint64_t)(int64_t)seed, 1);
+  HD_TRY(GatherSoA(ctx, ND, b, n_b, n_bp, shuffle, perm_b, s.bx, s.by, s.bz));
+  ScanArgs args = {};
+  args.a = a;
+  args.perm_a = MakeFeistel(n_a, (uint64_t)(int64_t)seed, 0);
+  args.order_mode = shuffle ? kOrderFeistel : kOrderIdentity;
+  args.pos_end = n_a;
+  args.bx = s.bx;
+  args.by = s.by;
+  args.bz = s.bz;
+  args.n_steps = n_bp / 64u;
+  args.seg_steps = args.n_steps;
+  args.n_segs = 1;
+  args.budget_steps = 0xFFFFFFFFu;
+  args.key = s.key;
+  args.cursors = s.cursors;
+  args.esc_idx = s.esc_idx;
+  args.esc_min = s.esc_min;
+  args.esc_count = s.counters + 1;
+  args.evals = s.key + 1;
+  HD_TRY(LaunchScan<ND>(ctx, args, 0u, n_a));
+  unsigned long long h[2] = {0, 0};
+  HD_HIP_TRY(hipMemcpyAsync(h, s.key, sizeof(h), hipMemcpyDeviceToHost,
+                            ctx->stream));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  std::memset(out, 0, sizeof(*out));
+  out->variant_used = HD_VARIANT_EB;
+  out->evals = h[1];
+  out->cmax2_bits = (uint32_t)(h[0] >> 32);
+  float c;
+  std::memcpy(&c, &out->cmax2_bits, 4);
+  out->hd = std::sqrt(c);
+  out->argmax = out->nn = 0xFFFFFFFFu;
+  if (h[0] == 0ull) return HD_OK;
+  out->argmax = 0xFFFFFFFFu - (uint32_t)h[0];
+  HD_TRY(LaunchNnIndex(ctx, ND, a, out->argmax, b, n_b, out->cmax2_bits, s.nn));
+  uint32_t nn = 0xFFFFFFFFu;
+  HD_HIP_TRY(hipMemcpyAsync(&nn, s.nn, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  out->nn = nn;
+  return HD_OK;
+}
+
+}  // namespace
+
+int EbSinglePassScratch(hd_context* ctx, int n_dims, const float* a,
+                        uint32_t n_a, const float* b, uint32_t n_b,
+                        bool shuffle, int32_t seed, const EbScratch& s,
+                        hd_directed_result* out) {
+  if (n_dims == 2)
+    return EbSinglePassScratchImpl<2>(ctx, a, n_a, b, n_b, shuffle, seed, s, out);
+  return EbSinglePassScratchImpl<3>(ctx, a, n_a, b, n_b, shuffle, seed, s, out);
+}
 
 int RunEarlyBreak(hd_context* ctx, const DirectedArgs& args,
                   hd_directed_result* out) {

@@ -281,6 +281,33 @@ int FinishDirected(hd_context* ctx, int n_dims, const float* a,
                    const unsigned long long* evals_dev, int n_counters,
                    hd_directed_result* out);
 
+// d_out[0] = the smallest index j of b with FP32 d2(a[argmax], b[j]) bits
+// equal to cmax2_bits (UINT32_MAX if none); asynchronous, allocates nothing.
+int LaunchNnIndex(hd_context* ctx, int n_dims, const float* a,
+                  uint32_t argmax, const float* b, uint64_t n_b,
+                  uint32_t cmax2_bits, uint32_t* d_out);
+
+// Device memory of the single-pass early break when the caller provides it
+// (the allocation-free entry point): no buffer is taken from the context.
+struct EbScratch {
+  unsigned long long* key;    // key, then the evaluation counter
+  unsigned int* counters;     // 4: cursor of the round, escalation count, ...
+  unsigned int* cursors;      // 1: the scan's segment cursor
+  uint32_t* nn;               // 1
+  uint32_t* esc_idx;          // 64 (never written: the budget covers all B)
+  float* esc_min;             // 64
+  float* bx;                  // max_bp each: B repacked (SoA, padded)
+  float* by;
+  float* bz;                  // 3D only
+  uint32_t max_bp;
+};
+// One early-break pass over A in the visit order (HD_FLAG_SINGLE_PASS) with
+// every buffer in `s`; synchronous; fills *out.
+int EbSinglePassScratch(hd_context* ctx, int n_dims, const float* a,
+                        uint32_t n_a, const float* b, uint32_t n_b,
+                        bool shuffle, int32_t seed, const EbScratch& s,
+                        hd_directed_result* out);
+
 // Input statistics of the stats JSON (stats.hip): one point set on the
 // device, AoS.
 struct StatsInput {

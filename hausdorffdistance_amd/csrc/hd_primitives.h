@@ -43,9 +43,11 @@ int GatherSoA(hd_context* ctx, int n_dims, const float* aos, uint32_t n,
               float* x, float* y, float* z);
 
 // Reads the MBR back (synchronous) and applies the reference's
-// ToNonemptyMBR widening (src/geoms/mbr.h:288-296).
+// ToNonemptyMBR widening (src/geoms/mbr.h:288-296). Non-finite coordinates
+// are an HD_EINVAL error. d_work: 8 device words, or null for a context
+// buffer.
 int HostMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
-            float* mbr /* host, 2*n_dims */);
+            float* mbr /* host, 2*n_dims */, uint32_t* d_work = nullptr);
 
 }  // namespace hd
 

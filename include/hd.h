@@ -203,6 +203,24 @@ int hd_directed_host(int n_dims, const float* a, uint64_t n_a, const float* b,
                      uint64_t n_b, uint32_t flags, int32_t seed,
                      int32_t parallelism, hd_directed_result* out);
 
+/* Allocation-free directed h(A->B) on device-resident AoS inputs: no
+ * counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The early break of hd_directed_device with
+ * HD_FLAG_SINGLE_PASS and the same kernels, but every buffer (key, counters,
+ * B repacked) lives in one device array of the library, 12 MiB + 1 KiB per
+ * GPU, held until the process ends; the call makes no allocation of any
+ * kind. Limits: |B| <= 1,048,576 points (HD_EINVAL above); ctx must be a
+ * context made beforehand (NULL is HD_EINVAL); flags take NO_SHUFFLE,
+ * SINGLE_PASS (implied) and COUNT_EVALS; LEARNED_ORDER is HD_EUNSUPPORTED.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (several threads, several contexts) race and must be serialised by the
+ * caller. Synchronous; same cmax2 bits, arg-max and NN as hd_directed_device. */
+int hd_directed_device_noalloc(hd_context* ctx, int n_dims, const float* d_a,
+                               uint64_t n_a, const float* d_b, uint64_t n_b,
+                               uint32_t flags, int32_t seed,
+                               hd_directed_result* out);
+
 /* Exact nearest neighbour of every query in `ref` on the device: the
  * per-point minimum the EB loop computes when it does not break. index:
  * HD_VARIANT_EB (exhaustive scan), HD_VARIANT_GRID or HD_VARIANT_LBVH.
