@@ -764,6 +764,22 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
   return HD_OK;
 }
 
+int hd_filter_probe_device(hd_context* ctx, int n_dims, const float* d_a,
+                           uint64_t n_a, const float* d_b, uint64_t n_b,
+                           float* d_out, float* d_cna, float* params) {
+  ClearError();
+  if (n_dims != 2 && n_dims != 3)
+    return Fail(HD_EINVAL, "n_dims must be 2 or 3, got %d", n_dims);
+  if (!d_a || !d_b || !d_out || !params || (n_dims == 3 && !d_cna))
+    return Fail(HD_EINVAL, "null pointer");
+  if (n_a == 0 || n_b == 0 || n_a > (1u << 16) || n_b > (1u << 16))
+    return Fail(HD_EINVAL, "the probe takes 1..65536 points per set");
+  ScopedContext sc;
+  HD_TRY(AcquireContext(ctx, &sc));
+  return FilterProbe(sc.ctx, n_dims, d_a, (uint32_t)n_a, d_b, (uint32_t)n_b,
+                     d_out, d_cna, params);
+}
+
 int hd_nn_device(hd_context* ctx, int index, int n_dims, const float* d_q,
                  uint64_t n_q, const float* d_ref, uint64_t n_ref,
                  float* d_out_d2, uint32_t* d_out_idx) {

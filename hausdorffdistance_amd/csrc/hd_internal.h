@@ -308,6 +308,12 @@ int EbSinglePassScratch(hd_context* ctx, int n_dims, const float* a,
                         bool shuffle, int32_t seed, const EbScratch& s,
                         hd_directed_result* out);
 
+// The matrix-core filter's D for every pair (test hook, eb.hip): B in
+// input order; params = centre[3], error bound E, 3D offset C.
+int FilterProbe(hd_context* ctx, int n_dims, const float* a, uint32_t n_a,
+                const float* b, uint32_t n_b, float* out, float* cna_out,
+                float* params);
+
 // Input statistics of the stats JSON (stats.hip): one point set on the
 // device, AoS.
 struct StatsInput {

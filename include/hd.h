@@ -282,6 +282,18 @@ int hd_mlp_loss_grad_device(hd_context* ctx, int n_dims, const float* params,
                             const float* x, const float* y, uint32_t batch,
                             float* loss, float* grad);
 
+/* Test hook of the early break's matrix-core filter (no reference
+ * counterpart): for device AoS sets A and B (1..65536 points each), the
+ * filter's approximate value D for every pair, computed exactly as the
+ * filter does (centred 16-point tiles, v_mfma_f32_16x16x4_f32):
+ * d_out[i * n_b + j] approximates d2(a_i, b_j) in 2D; in 3D it is
+ * d2 + d_cna[i] (d_cna may be NULL in 2D). params (host, 5 floats) receives
+ * the centre (3), the error bound E the filter assumes for |D - d2| (3D:
+ * |D - cna - d2|, before its threshold arithmetic) and the 3D offset C. */
+int hd_filter_probe_device(hd_context* ctx, int n_dims, const float* d_a,
+                           uint64_t n_a, const float* d_b, uint64_t n_b,
+                           float* d_out, float* d_cna, float* params);
+
 /* Host-side pieces of the input statistics (stats_n_points_cell), as test
  * hooks: for the multiset whose value v occurs hist[v] times (the number of
  * grid cells holding v points), the Gini index

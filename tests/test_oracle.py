@@ -185,6 +185,28 @@ def test_hd_bounds_bracket_the_distance():
         assert lb2 <= r.cmax2 * (1 + 1e-6) and r.cmax2 <= ub2 * (1 + 1e-6)
 
 
+def test_hd_bounds_by_hand():
+    """hd_bounds.h:123-160 worked by hand for A = [0,1]^2, B = [3,4] x [0,1].
+    Lower bound: the largest face-to-box min distance, A's left face x = 0
+    is 3 from B: LB^2 = 9. Upper bound: per face of A, the smallest
+    face-to-face max distance over B's faces (per axis the larger of
+    |lo1 - hi2| and |hi1 - lo2|), scanned bottom, right, top, left and cut
+    short once a value cannot raise the running maximum:
+      bottom: 16, 17, 17, 10 -> 10;  right: 10 <= 10, cut, no candidate;
+      top: 17, 17, 16, then 10 <= 10, cut -> 16;
+      left: 17, 17, 17, then 10 <= 16, cut -> 17.
+    UB^2 = 17. A bound that took the smallest max over all faces (5, from
+    A's right face to B's left face) or the largest (17 or 18) for every
+    face would differ."""
+    lb2, ub2 = oracle.hd_bounds2(np.array([0, 0, 1, 1], np.float32),
+                                 np.array([3, 0, 4, 1], np.float32), 2)
+    assert lb2 == np.float32(9) and ub2 == np.float32(17)
+    # Swapped roles: B's right face x = 4 is 3 from A.
+    lb2, ub2 = oracle.hd_bounds2(np.array([3, 0, 4, 1], np.float32),
+                                 np.array([0, 0, 1, 1], np.float32), 2)
+    assert lb2 == np.float32(9) and ub2 == np.float32(17)
+
+
 def test_mbr_matches_numpy():
     p = gen.uniform(70, 1000, 3) - np.float32(0.5)
     m = oracle.mbr(p)
@@ -224,3 +246,31 @@ def test_mlp_gradient_matches_finite_differences():
         dp = float(np.float32(pp[k])) - float(np.float32(pm[k]))
         fd = (lp - lm) / dp
         assert abs(fd - g[k]) <= 2e-3 * max(1.0, abs(g[k])), (k, fd, g[k])
+
+
+def test_fp32_d2_exceeds_the_real_distance_by_under_2_pow_minus_19():
+    """The arithmetic behind the filter's escalation bound: a partial minimum
+    escalated from the matrix-core filter is bounded by approx + E, a bound
+    of the *real* |a - b|^2, and a later "partial minimum below cmax2" skip
+    compares it with FP32 d2 values. The reference's FP32 formula rounds the
+    two differences, the two squares and the sum (5 roundings), so it can
+    exceed the real value by about 5u relative (u = 2^-24): more than the
+    2u of slack the bound E leaves when d2 is near rho^2, and well below the
+    2^-19 (32u) widening the bound gets. Found here on pairs chosen to round
+    upward at every step, measured against exact rationals."""
+    u = Fraction(1, 2 ** 24)
+    rng = np.random.default_rng(19)
+    worst = Fraction(0)
+    for _ in range(4000):
+        # coordinates whose differences are not representable (mixed
+        # exponents) so that every operation rounds
+        p = (rng.uniform(-1, 1, 2) * np.float32(2.0) ** rng.integers(-3, 1, 2)).astype(np.float32)
+        q = (rng.uniform(-1, 1, 2) * np.float32(2.0) ** rng.integers(-3, 1, 2)).astype(np.float32)
+        real = sum((Fraction(float(a)) - Fraction(float(b))) ** 2 for a, b in zip(p, q))
+        if real == 0:
+            continue
+        got = Fraction(float(oracle.d2(p, q)))
+        worst = max(worst, (got - real) / real)
+    assert worst > 2 * u          # the plain E slack would not cover it
+    assert worst < 6 * u          # the 5-rounding analysis
+    assert 6 * u < Fraction(1, 2 ** 19)

@@ -136,6 +136,8 @@ def test_hd_bounds_library_matches_oracle():
     from oracle import oracle
     # Boxes side by side: the farthest face of A (x = 0) is 3 away from B.
     assert lib_bounds([0, 0, 1, 1], [3, 0, 4, 1], 2)[0] == np.float32(3)
+    # and its upper bound, sqrt(17) (worked by hand in test_oracle.py)
+    assert lib_bounds([0, 0, 1, 1], [3, 0, 4, 1], 2)[1] == np.sqrt(np.float32(17))
     rng = np.random.default_rng(8)
     for nd in (2, 3):
         for _ in range(200):
