@@ -80,7 +80,9 @@ class RunConfig:
     n_points_cell: int = 8
     max_hit: int = 256
     eb_only_threshold: int = 0
-    auto_tune: bool = False
+    auto_tune: bool = False            # strategy and parameters by trial
+    auto_tune_n_points_cell: bool = False
+    auto_tune_max_hit: bool = False
     stats_n_points_cell: int = 10  # > 0: input statistics in the stats JSON
 
     def to_c(self) -> _lib.RunConfigC:
@@ -106,7 +108,9 @@ class RunConfig:
         c.n_points_cell = int(self.n_points_cell)
         c.max_hit = int(self.max_hit)
         c.eb_only_threshold = int(self.eb_only_threshold)
-        c.auto_tune = 1 if self.auto_tune else 0
+        c.auto_tune = ((_lib.HD_AUTO_TUNE_ALL if self.auto_tune else 0)
+                       | (_lib.HD_AUTO_TUNE_N_POINTS_CELL if self.auto_tune_n_points_cell else 0)
+                       | (_lib.HD_AUTO_TUNE_MAX_HIT if self.auto_tune_max_hit else 0))
         c.stats_n_points_cell = int(self.stats_n_points_cell)
         return c
 

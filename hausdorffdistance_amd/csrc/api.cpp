@@ -650,9 +650,12 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
     hd_directed_result rab, rba;
     std::memset(&rba, 0, sizeof(rba));
     HD_TRY(TimedDirected(c, cfg->variant, ab, &rab));
+    const TunedChoice choice_ab = c->last_choice;
+    TunedChoice choice_ba;
     double t = rab.time_ms;
     if (symmetric) {
       HD_TRY(TimedDirected(c, cfg->variant, ba, &rba));
+      choice_ba = c->last_choice;
       t += rba.time_ms;
     }
     times.push_back(t);
@@ -666,7 +669,28 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
          << ", \"CMax2Bits\": " << rab.cmax2_bits
          << ", \"ArgMax\": " << rab.argmax
          << ", \"HDLowerBound\": " << Num(hd_lb)
-         << ", \"HDUpperBound\": " << Num(hd_ub) << "}";
+         << ", \"HDUpperBound\": " << Num(hd_ub);
+    if (cfg->variant == HD_VARIANT_HYBRID) {
+      // The hybrid's parameters (the reference writes the predicted ones,
+      // hausdorff_distance_hybrid.h:744-776): per direction, and the A->B
+      // ones under the reference's keys.
+      auto choice_json = [](const TunedChoice& ch) {
+        std::ostringstream o;
+        o << "{\"Variant\": \"" << AlgorithmName(ch.variant)
+          << "\", \"NumPointsPerCell\": " << ch.n_points_cell
+          << ", \"MaxHit\": " << ch.max_hit
+          << ", \"Tuned\": " << (ch.tuned ? "true" : "false")
+          << ", \"Trials\": " << ch.trials << "}";
+        return o.str();
+      };
+      if (choice_ab.tuned)
+        reps << ", \"PredictedNPointsCell\": " << choice_ab.n_points_cell
+             << ", \"PredictedMaxHit\": " << choice_ab.max_hit;
+      reps << ", \"HybridChoice\": {\"AB\": " << choice_json(choice_ab);
+      if (symmetric) reps << ", \"BA\": " << choice_json(choice_ba);
+      reps << "}";
+    }
+    reps << "}";
   }
   // Result.
   out->n_repeats = (uint32_t)cfg->repeat;

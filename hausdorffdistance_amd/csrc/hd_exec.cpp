@@ -59,7 +59,9 @@ struct Flags {
 };
 
 const char* kBoolFlags[] = {"check", "normalize", "symmetric", "learned_order",
-                            "auto_tune", "overwrite", "shuffle", "vary_params"};
+                            "auto_tune", "auto_tune_n_points_cell",
+                            "auto_tune_max_hit", "overwrite", "shuffle",
+                            "vary_params"};
 
 bool IsBool(const std::string& k) {
   for (const char* b : kBoolFlags)
@@ -272,7 +274,8 @@ int main(int argc, char** argv) {
             "Usage: hd_exec -input1 A -input2 B -input_type wkt|off|ply|bin|synthetic "
             "-n_dims 2|3 [-variant eb|grid|rt|lbvh|hybrid|nn|rt-hdist] [-execution gpu|cpu] "
             "[-repeat 5] [-seed 0] [-parallelism -1] [-check|-nocheck] [-symmetric] [-learned_order] "
-            "[-n_points_cell 8] [-max_hit 256] [-auto_tune] [-limit N] "
+            "[-n_points_cell 8] [-max_hit 256] [-auto_tune] [-auto_tune_n_points_cell] "
+            "[-auto_tune_max_hit] [-limit N] "
             "[-stats_n_points_cell 10] [-vary_params -sample_rate_list r,.. "
             "-eb_only_threshold_list n,.. -max_hit_list n,.. -n_points_cell_list n,.. "
             "[-overwrite]] "
@@ -313,7 +316,9 @@ int main(int argc, char** argv) {
   cfg.n_points_cell = (uint32_t)f.Int("n_points_cell", 8);
   cfg.max_hit = (uint32_t)f.Int("max_hit", 256);
   cfg.eb_only_threshold = (uint32_t)f.Int("eb_only_threshold", 0);
-  cfg.auto_tune = f.Bool("auto_tune", false) ? 1 : 0;
+  cfg.auto_tune = (f.Bool("auto_tune", false) ? HD_AUTO_TUNE_ALL : 0) |
+                  (f.Bool("auto_tune_n_points_cell", false) ? HD_AUTO_TUNE_N_POINTS_CELL : 0) |
+                  (f.Bool("auto_tune_max_hit", false) ? HD_AUTO_TUNE_MAX_HIT : 0);
   cfg.stats_n_points_cell = (int32_t)f.Int("stats_n_points_cell", 10);
   // -check defaults to on, as the reference's flag (src/flags.cc:49): every
   // run is compared with the host early break (-nocheck turns it off).

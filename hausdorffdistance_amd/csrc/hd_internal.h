@@ -51,6 +51,16 @@ namespace dev {
 using hd::Feistel;
 }
 
+// Strategy and index parameters the hybrid runs with (chosen by rules, by
+// the sampled pass or by trial under auto_tune).
+struct TunedChoice {
+  int variant = -1;          // HD_VARIANT_EB / GRID / LBVH
+  uint32_t n_points_cell = 0;
+  uint32_t max_hit = 0;
+  bool tuned = false;        // chosen by trial (auto_tune)
+  uint32_t trials = 0;       // runs timed to choose it (0: cached or rules)
+};
+
 // Grow-only device buffer cached in a context under a name.
 struct DeviceBuffer {
   void* ptr = nullptr;
@@ -97,8 +107,11 @@ struct hd_context {
   std::map<std::string, hd::StageStat> stage_stats;
   // Ordering model trained on demand for HD_FLAG_LEARNED_ORDER without an
   // explicit model, with the inputs it was trained for.
-  // Hybrid selections made by trial (auto_tune), keyed by input signature.
-  std::map<std::string, int> tuned_variant;
+  // Hybrid selections made by trial (auto_tune), keyed by a signature of
+  // the inputs' content (sizes, MBRs, a strided sample of the points), and
+  // the choice of the last hybrid call.
+  std::map<std::string, hd::TunedChoice> tuned;
+  hd::TunedChoice last_choice;
   // Batched-mode plan and HIP graph for the last batch signature.
   hd::BatchPlan* batch_plan = nullptr;
   hd_mlp* cached_model = nullptr;

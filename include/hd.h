@@ -89,6 +89,15 @@ extern "C" {
                                         variant only: a comparison
                                         baseline, same result */
 
+/* hd_run_config.auto_tune (reference -auto_tune, -auto_tune_n_points_cell,
+ * -auto_tune_max_hit, src/flags.cc:28-31; the reference predicts them with
+ * gradient-boosted trees, hausdorff_distance_hybrid.h:722-778). ALL also
+ * picks the strategy (grid with early break, LBVH or the early-break scan).
+ * The choice is cached per context for inputs with the same content. */
+#define HD_AUTO_TUNE_ALL 0x1
+#define HD_AUTO_TUNE_N_POINTS_CELL 0x2
+#define HD_AUTO_TUNE_MAX_HIT 0x4
+
 /* Reference RunConfig (src/run_config.h:20-61) restricted to the hot path,
  * with the defaults of src/flags.cc:3-57 (hd_run_config_init). */
 typedef struct hd_run_config {
@@ -106,7 +115,10 @@ typedef struct hd_run_config {
                               is handed to the EB kernel (-max_hit, 256) */
   uint32_t eb_only_threshold; /* hybrid: EB only if sampled p80 iterations
                               fall below this (-eb_only_threshold, 0) */
-  int32_t auto_tune;       /* 1: parameters from the autotuner (-auto_tune) */
+  int32_t auto_tune;       /* hybrid parameters chosen per input by timed
+                              trials instead of the flags: HD_AUTO_TUNE_*
+                              bits (-auto_tune, -auto_tune_n_points_cell,
+                              -auto_tune_max_hit) */
   int32_t stats_n_points_cell; /* points per cell of the input-statistics
                               grid written to the stats JSON ("Input":
                               MBR, density, cell counts, Gini, percentiles;

@@ -114,13 +114,38 @@ def test_nearest_neighbour_entry_point(ctx, index, n_dims):
 
 
 def test_hybrid_auto_tune_keeps_bits(ctx):
-    a, b = hd.gen.gmm_pair(100_000)
+    """auto_tune picks the strategy, n_points_cell and max_hit by trial on
+    the input (the reference predicts them, hausdorff_distance_hybrid.h:
+    722-778) and caches the choice under the input's content: the second
+    call reuses it, other data at the same addresses is tuned anew, and the
+    bits never change."""
+    a, b = hd.gen.gmm_pair(300_000)
     o_ab, o_ba = oracle.eb_symmetric(a, b)
-    for _ in range(2):  # first call tunes by trial, second uses the choice
-        r = run(a, b, hd.Variant.HYBRID, ctx=ctx, auto_tune=True)
+    for call in range(2):  # the first call tunes by trial, the second reuses
+        r = run(a, b, hd.Variant.HYBRID, ctx=ctx, auto_tune=True, stats_n_points_cell=0)
         assert triple(r.ab) == triple(o_ab)
         assert triple(r.ba) == triple(o_ba)
-        assert r.ab.variant_used in (0, 1, 2)
+        rep = r.stats["Running"]["Repeats"][0]
+        ch = rep["HybridChoice"]["AB"]
+        print(f"[tune] call {call}: {rep['HybridChoice']}", flush=True)
+        assert ch["Tuned"] and ch["Variant"] in ("Uniform Grid", "LBVH", "Early Break")
+        assert ch["NumPointsPerCell"] in (2, 4, 8, 16, 32)
+        assert ch["MaxHit"] in (64, 256, 1024, 4096)
+        assert rep["PredictedNPointsCell"] == ch["NumPointsPerCell"]
+        assert (ch["Trials"] > 1) == (call == 0)
+    # Other points of the same sizes, uploaded to the same context buffers:
+    # a new signature, so a new trial.
+    a2 = (a * np.float32(0.5)).astype(np.float32)
+    r = run(a2, b, hd.Variant.HYBRID, ctx=ctx, auto_tune=True, stats_n_points_cell=0,
+            symmetric=False)
+    assert r.stats["Running"]["Repeats"][0]["HybridChoice"]["AB"]["Trials"] > 1
+    assert triple(r.ab) == triple(oracle.eb_directed(a2, b))
+    # Only n_points_cell: the grid path, one parameter tried.
+    r = run(a, b, hd.Variant.HYBRID, ctx=ctx, auto_tune_n_points_cell=True,
+            stats_n_points_cell=0, symmetric=False)
+    ch = r.stats["Running"]["Repeats"][0]["HybridChoice"]["AB"]
+    assert ch["Variant"] == "Uniform Grid" and ch["MaxHit"] == 256 and ch["Trials"] == 5
+    assert triple(r.ab) == triple(o_ab)
 
 
 def test_hybrid_escalates_far_queries(ctx):
