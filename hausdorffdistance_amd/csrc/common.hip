@@ -154,6 +154,16 @@ __global__ void NnIndexKernel(const float* __restrict__ a, uint32_t argmax,
   if (LaneId() == 0 && best != 0xFFFFFFFFu) atomicMin(out, best);
 }
 
+// out[k] = point floor(k * n / m) of pts, k < m (AoS, nd floats each).
+__global__ void SamplePointsKernel(const float* __restrict__ pts, uint64_t n,
+                                   int nd, uint32_t m, float* __restrict__ out) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m;
+       k += gridDim.x * blockDim.x) {
+    const uint64_t src = (uint64_t)k * n / m;
+    for (int d = 0; d < nd; ++d) out[(uint64_t)k * nd + d] = pts[src * nd + d];
+  }
+}
+
 inline int GridFor(uint64_t n, int threads, int cap) {
   uint64_t b = (n + threads - 1) / threads;
   if (b < 1) b = 1;
@@ -177,6 +187,15 @@ Feistel MakeFeistel(uint32_t n, uint64_t seed, uint32_t stream_id) {
 }
 
 uint32_t HostPermute(const Feistel& f, uint32_t i) { return dev::Permute(f, i); }
+
+int SamplePoints(hd_context* ctx, const float* pts, uint64_t n, int n_dims,
+                 uint32_t m, float* d_out) {
+  if (n == 0 || m == 0) return HD_OK;
+  SamplePointsKernel<<<(m + 255) / 256, 256, 0, ctx->stream>>>(pts, n, n_dims,
+                                                               m, d_out);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
 
 int FillU32(hd_context* ctx, uint32_t* p, uint32_t value, uint64_t n) {
   if (n == 0) return HD_OK;

@@ -27,6 +27,10 @@ Feistel MakeFeistel(uint32_t n, uint64_t seed, uint32_t stream_id);
 // The same permutation evaluated on the host: position i -> index.
 uint32_t HostPermute(const Feistel& f, uint32_t i);
 
+// d_out[k] = point floor(k n / m) of the AoS set pts, for k < m.
+int SamplePoints(hd_context* ctx, const float* pts, uint64_t n, int n_dims,
+                 uint32_t m, float* d_out);
+
 // p[i] = i for i < n.
 int IotaU32(hd_context* ctx, uint32_t* p, uint32_t n);
 

@@ -140,11 +140,16 @@ def test_hybrid_auto_tune_keeps_bits(ctx):
             symmetric=False)
     assert r.stats["Running"]["Repeats"][0]["HybridChoice"]["AB"]["Trials"] > 1
     assert triple(r.ab) == triple(oracle.eb_directed(a2, b))
-    # Only n_points_cell: the grid path, one parameter tried.
+    # Only n_points_cell, from a poor starting value (cells of ~4096
+    # points): the grid path, the candidates 2..32 tried, and one of them
+    # replaces the flag's value.
     r = run(a, b, hd.Variant.HYBRID, ctx=ctx, auto_tune_n_points_cell=True,
-            stats_n_points_cell=0, symmetric=False)
-    ch = r.stats["Running"]["Repeats"][0]["HybridChoice"]["AB"]
-    assert ch["Variant"] == "Uniform Grid" and ch["MaxHit"] == 256 and ch["Trials"] == 5
+            n_points_cell=4096, stats_n_points_cell=0, symmetric=False)
+    rep = r.stats["Running"]["Repeats"][0]
+    ch = rep["HybridChoice"]["AB"]
+    print(f"[tune] n_points_cell from 4096: {ch}", flush=True)
+    assert ch["Variant"] == "Uniform Grid" and ch["MaxHit"] == 256 and ch["Trials"] == 6
+    assert rep["PredictedNPointsCell"] in (2, 4, 8, 16, 32)
     assert triple(r.ab) == triple(o_ab)
 
 
