@@ -95,6 +95,7 @@ struct ScanArgs {
   uint32_t seg_steps;     // steps per segment
   uint32_t n_segs;        // segments; 1 = whole-B scans
   uint32_t budget_steps;  // n_segs == 1: steps before a slot escalates
+  uint32_t start_step;    // n_segs == 1: the step every wave starts at
   unsigned long long* key;
   unsigned int* cursors;  // per segment: next order position
   uint32_t claim;         // positions taken per cursor claim (<= 64)
@@ -241,9 +242,11 @@ HD_DEV void ScanSegment(const ScanArgs& args, uint32_t seg, uint32_t lane,
 
   w.live = 0;
   w.gstep = 0;
-  // Waves start at the segment's first step and walk B together (shared
-  // L2 lines; 10M GMM: L2 hit rate 0.27 -> 0.66).
-  w.step = seg_begin;
+  // All waves start at the same step and walk B together (shared L2
+  // lines; 10M GMM: L2 hit rate 0.27 -> 0.66): the segment's first step,
+  // or a per-launch step for whole-B scans, so that successive rounds do
+  // not all re-scan the same first steps.
+  w.step = split ? seg_begin : seg_begin + args.start_step % seg_len;
   uint32_t next_due = 0xFFFFFFFFu;
   // Reservoir: one candidate per lane, claimed 64 positions at a time.
   float rp[ND];
@@ -510,6 +513,7 @@ struct MfmaArgs {
   const float* bz;
   uint32_t n_steps;
   uint32_t budget_steps;  // < n_steps: no slot ever completes a full scan
+  uint32_t start_step;    // n_segs == 1: the step every wave starts at
   float cx, cy, cz;       // centre
   float err;              // E
   float off;              // 3D: C
@@ -693,7 +697,7 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
     // B in roughly the same window, so the L2 of each XCD serves most of
     // the tile loads (10M GMM: L2 hit rate 0.12 with scattered starts,
     // 0.55 aligned; 2-5% less time per launch).
-    uint32_t step = seg_begin;
+    uint32_t step = split ? seg_begin : seg_begin + args.start_step % seg_len;
     live = 0;
     rhead = rcount = 0;
     drained = false;
@@ -976,6 +980,17 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
   if (lane == 0) atomicAdd(args.evals, evals);
 }
 
+// The step at which every wave of a whole-B launch starts: fixed by the
+// launch's first position and budget, so that the rounds of one run start
+// at unrelated steps (an escalated point then does not re-scan the steps
+// its previous round covered).
+inline uint32_t RoundStartStep(uint32_t pos_begin, uint32_t budget,
+                               uint32_t n_steps) {
+  const uint64_t h = (uint64_t)pos_begin * 0x9E3779B97F4A7C15ull +
+                     (uint64_t)budget * 0xC2B2AE3D27D4EB4Full;
+  return n_steps ? (uint32_t)((h >> 32) % n_steps) : 0u;
+}
+
 constexpr int kP = 8;  // slots per wave
 constexpr int kS = 4;  // B steps per iteration
 
@@ -1092,6 +1107,7 @@ int LaunchSlots(hd_context* ctx, const float* a, int order_mode,
   args.seg_steps = n_steps;
   args.n_segs = 1;
   args.budget_steps = budget_steps;
+  args.start_step = RoundStartStep(pos_begin, budget_steps, n_steps);
   args.key = key;
   args.esc_idx = esc_idx;
   args.esc_min = esc_min;
@@ -1204,6 +1220,7 @@ int LaunchMfma(hd_context* ctx, const float* a, int order_mode,
   args.bz = t.bz;
   args.n_steps = t.n_steps;
   args.budget_steps = budget_steps;
+  args.start_step = RoundStartStep(pos_begin, budget_steps, t.n_steps);
   args.cx = t.center[0];
   args.cy = t.center[1];
   args.cz = t.center[2];

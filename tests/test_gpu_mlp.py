@@ -156,6 +156,8 @@ def test_learned_order_evaluates_fewer_pairs(ctx):
         assert (r.cmax2_bits, r.argmax, r.nn) == want, name
         got[name] = r.evals
     print(f"[mlp] evaluations: {got}", flush=True)
-    assert got["learned_single"] < got["plain_single"]
-    assert got["learned"] < got["plain"]
+    # Measured on one MI355X: 6.5e10 against 1.02e11 as a single pass
+    # (0.64x), 1.36e10 against 2.77e10 through the pipeline (0.49x).
+    assert got["learned_single"] < 0.8 * got["plain_single"]
+    assert got["learned"] < 0.7 * got["plain"]
     model.close()
