@@ -118,6 +118,8 @@ SIGNATURES = {
     "hd_context_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "hd_context_destroy": (None, [_P]),
     "hd_context_stream": (C.c_void_p, [_P]),
+    "hd_context_set_memory_limit": (C.c_int, [_P, C.c_uint64]),
+    "hd_context_memory_held": (C.c_uint64, [_P]),
     "hd_run_hausdorff_distance": (
         C.c_int,
         [_P, C.POINTER(RunConfigC), _P, C.c_uint64, _P, C.c_uint64, C.POINTER(ResultC),
@@ -135,6 +137,8 @@ SIGNATURES = {
     ),
     "hd_filter_probe_device": (
         C.c_int, [_P, C.c_int, _P, C.c_uint64, _P, C.c_uint64, _P, _P, _F]),
+    "hd_sort_pairs_probe_device": (
+        C.c_int, [_P, C.c_int, _P, _P, C.c_uint32, C.c_int, _P, _P]),
     "hd_directed_host": (
         C.c_int,
         [C.c_int, _P, C.c_uint64, _P, C.c_uint64, C.c_uint32, C.c_int32, C.c_int32,

@@ -170,6 +170,17 @@ class Context:
     def stream(self) -> int:
         return int(_lib.load().hd_context_stream(self.handle) or 0)
 
+    def set_memory_limit(self, nbytes: int) -> None:
+        """Caps the device bytes this context's work buffers may hold (0: no
+        cap) and releases the buffers held now; a call that needs more
+        raises HDError(HD_ENOMEM) and leaves the context usable."""
+        _lib.check(_lib.load().hd_context_set_memory_limit(self.handle, int(nbytes)))
+
+    @property
+    def memory_held(self) -> int:
+        """Device bytes the context's work buffers hold now."""
+        return int(_lib.load().hd_context_memory_held(self.handle))
+
     def set_profiling(self, enable: bool) -> None:
         """Per-stage device timing on this context's stream."""
         _lib.check(_lib.load().hd_context_set_profiling(self.handle, 1 if enable else 0))

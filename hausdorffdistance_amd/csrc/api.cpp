@@ -292,14 +292,21 @@ int ContextBuffer(hd_context* ctx, const char* name, size_t bytes,
                   void** out) {
   DeviceBuffer& b = ctx->buffers[name];
   if (b.bytes < bytes) {
+    const size_t want = std::max<size_t>(bytes, 256);
+    if (ctx->memory_limit && ctx->memory_held - b.bytes + want > ctx->memory_limit)
+      return Fail(HD_ENOMEM,
+                  "context memory limit of %llu bytes reached: %s needs %zu bytes, "
+                  "%llu held by other buffers",
+                  (unsigned long long)ctx->memory_limit, name, want,
+                  (unsigned long long)(ctx->memory_held - b.bytes));
     if (b.ptr) {
       // The stream may still use the old buffer.
       HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
       HD_HIP_TRY(hipFree(b.ptr));
+      ctx->memory_held -= b.bytes;
       b.ptr = nullptr;
       b.bytes = 0;
     }
-    size_t want = std::max<size_t>(bytes, 256);
     hipError_t e = hipMalloc(&b.ptr, want);
     if (e != hipSuccess) {
       b.ptr = nullptr;
@@ -308,6 +315,7 @@ int ContextBuffer(hd_context* ctx, const char* name, size_t bytes,
                   hipGetErrorString(e));
     }
     b.bytes = want;
+    ctx->memory_held += want;
   }
   *out = b.ptr;
   return HD_OK;
@@ -486,6 +494,26 @@ void hd_context_destroy(hd_context* ctx) {
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
+
+int hd_context_set_memory_limit(hd_context* ctx, uint64_t bytes) {
+  ClearError();
+  if (!ctx) return Fail(HD_EINVAL, "null context");
+  HD_TRY(ContextActivate(ctx));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  // The batch plan's graph holds pointers into the buffers released below.
+  if (ctx->batch_plan) {
+    DestroyBatchPlan(ctx->batch_plan);
+    ctx->batch_plan = nullptr;
+  }
+  for (auto& kv : ctx->buffers)
+    if (kv.second.ptr) HD_HIP_TRY(hipFree(kv.second.ptr));
+  ctx->buffers.clear();
+  ctx->memory_held = 0;
+  ctx->memory_limit = bytes;
+  return HD_OK;
+}
+
+uint64_t hd_context_memory_held(hd_context* ctx) { return ctx ? ctx->memory_held : 0; }
 
 int hd_context_set_profiling(hd_context* ctx, int enable) {
   ClearError();

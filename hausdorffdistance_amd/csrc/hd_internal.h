@@ -97,6 +97,9 @@ struct hd_context {
   int cu_count = 0;
   int max_waves_per_cu = 32;
   std::unordered_map<std::string, hd::DeviceBuffer> buffers;
+  // Device bytes the buffers hold, and the most they may hold (0: no limit).
+  uint64_t memory_held = 0;
+  uint64_t memory_limit = 0;
   void* host_pinned = nullptr;  // small pinned staging area
   size_t host_pinned_bytes = 0;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;

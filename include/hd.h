@@ -173,6 +173,18 @@ int hd_context_create(int device, hd_context** out);
 void hd_context_destroy(hd_context* ctx);
 /* The context's hipStream_t, as void*. */
 void* hd_context_stream(hd_context* ctx);
+/* Caps the device memory the context's work buffers may hold at `bytes`
+ * (0 = no cap) and releases the buffers it holds now (they are allocated
+ * again on demand; a cached batch plan is dropped with them). A call that
+ * would grow the buffers past the cap fails with HD_ENOMEM before it
+ * launches anything that needs the missing buffer, and leaves the context
+ * usable. Trained ordering models are not counted. No counterpart in the
+ * reference, which allocates through thrust and aborts when an allocation
+ * fails; it serves callers that share the GPU and must bound this library,
+ * and tests of the allocation-failure path. */
+int hd_context_set_memory_limit(hd_context* ctx, uint64_t bytes);
+/* Device bytes the context's work buffers hold now. */
+uint64_t hd_context_memory_held(hd_context* ctx);
 
 /* Replaces hd::RunHausdorffDistance (src/run_hausdorff_distance.cu:53-402)
  * for in-memory inputs: validates, uploads (unless INPUT_ON_DEVICE), runs
@@ -305,6 +317,17 @@ int hd_mlp_loss_grad_device(hd_context* ctx, int n_dims, const float* params,
 int hd_filter_probe_device(hd_context* ctx, int n_dims, const float* d_a,
                            uint64_t n_a, const float* d_b, uint64_t n_b,
                            float* d_out, float* d_cna, float* params);
+
+/* Test hook of the library's stable radix sort of (key, u32 value) pairs
+ * (the reference sorts Morton codes with thrust::sort_by_key,
+ * src/index/uniform_grid.h, src/lbvh/lbvh/bvh.cuh): sorts n device pairs on
+ * the context stream and waits. key_kind 0: u16 keys, ascending; 1: u32
+ * keys, ascending on bits [0, end_bit); 2: finite f32 keys, descending.
+ * Equal keys keep their input order. Outputs must not alias inputs. */
+int hd_sort_pairs_probe_device(hd_context* ctx, int key_kind,
+                               const void* d_keys_in, const uint32_t* d_vals_in,
+                               uint32_t n, int end_bit, void* d_keys_out,
+                               uint32_t* d_vals_out);
 
 /* Host-side pieces of the input statistics (stats_n_points_cell), as test
  * hooks: for the multiset whose value v occurs hist[v] times (the number of
