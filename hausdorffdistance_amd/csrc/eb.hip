@@ -486,6 +486,10 @@ __global__ void SeedListKernel(const uint32_t* order, Feistel perm,
 // chip and its drain is a quarter as long.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// B steps (64 points each) per filter iteration: drops, escalations and
+// refills are decided once per iteration.
+constexpr int kFilterSteps = 4;
+
 struct MfmaArgs {
   const float* a;         // AoS A (original coordinates)
   const uint32_t* order;
@@ -702,19 +706,18 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
     rhead = rcount = 0;
     drained = false;
     gstep = 0;
-    // An iteration covers two B steps (8 tiles, 32 MFMAs); the next
-    // iteration's tiles and original coordinates are requested at the top of
-    // each iteration.
+    // An iteration covers kFilterSteps B steps (4 kFilterSteps tiles, 16
+    // kFilterSteps MFMAs); the next iteration's tiles are requested at the
+    // top of each iteration.
     auto next_step = [&](uint32_t st) __attribute__((always_inline)) {
       return st + 1u == seg_end ? seg_begin : st + 1u;
     };
-    float bcur[8], ocur[2][ND];
-    {
-      const uint32_t st1 = next_step(step);
-      LoadStepTiles(args.bt, step, lane4, bcur);
-      LoadStepTiles(args.bt, st1, lane4, bcur + 4);
-      LoadStep<ND>(args.bx, args.by, args.bz, step, lane, ocur[0]);
-      LoadStep<ND>(args.bx, args.by, args.bz, st1, lane, ocur[1]);
+    float bcur[4 * kFilterSteps];
+    uint32_t nxt = step;  // first step of the next iteration
+#pragma unroll
+    for (int k = 0; k < kFilterSteps; ++k) {
+      LoadStepTiles(args.bt, nxt, lane4, bcur + 4 * k);
+      nxt = next_step(nxt);
     }
 
     while (true) {
@@ -820,19 +823,21 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
       // Segmented scans poll the other segments' verdicts every 8 steps
       // (issued before the next steps' loads: loads complete in order, so
       // waiting for the flag does not wait for them).
-      const bool poll = split && (gstep & 7u) == 6u;
+      const bool poll = split && ((gstep + kFilterSteps) & 7u) == 0u;
       uint32_t flag = 0;
       if (poll && ((live >> lane) & 1ull))
         flag = __hip_atomic_load(&args.broken[m_pos], __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT);
-      // Two steps: 8 B tiles against the 4 A tiles (32 MFMAs).
-      const uint32_t s2 = next_step(next_step(step));
-      const uint32_t s3 = next_step(s2);
-      float bnx[8], onx[2][ND];
-      LoadStepTiles(args.bt, s2, lane4, bnx);
-      LoadStepTiles(args.bt, s3, lane4, bnx + 4);
-      LoadStep<ND>(args.bx, args.by, args.bz, s2, lane, onx[0]);
-      LoadStep<ND>(args.bx, args.by, args.bz, s3, lane, onx[1]);
+      // The next iteration's tiles, then kFilterSteps steps: 4 kFilterSteps
+      // B tiles against the 4 A tiles (16 kFilterSteps MFMAs).
+      const uint32_t cur_step = step;
+      step = nxt;
+      float bnx[4 * kFilterSteps];
+  #pragma unroll
+      for (int k = 0; k < kFilterSteps; ++k) {
+        LoadStepTiles(args.bt, nxt, lane4, bnx + 4 * k);
+        nxt = next_step(nxt);
+      }
       int32_t sm[4];  // this iteration's approximate minima
       const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
   #pragma unroll
@@ -841,15 +846,15 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
         // A tile without live slots (the cursor has drained) costs nothing.
         if (((live >> (16 * t)) & 0xFFFFull) == 0ull) continue;
   #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < 4 * kFilterSteps; ++q) {
           const floatx4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(bcur[q], bop[t], zero, 0, 0, 0);
           sm[t] = Min3I(sm[t], __float_as_int(d[0]), __float_as_int(d[1]));
           sm[t] = Min3I(sm[t], __float_as_int(d[2]), __float_as_int(d[3]));
         }
         mn[t] = min(mn[t], sm[t]);
       }
-      gstep += 2;
-      evals += (unsigned long long)__popcll(live) * 128ull;
+      gstep += kFilterSteps;
+      evals += (unsigned long long)__popcll(live) * (64ull * kFilterSteps);
 
       // Drops (proved below cmax2, or verified exactly) and escalations.
       unsigned long long hit = 0, maybe = 0;
@@ -869,22 +874,34 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
       }
       hit &= live;
       maybe &= live & ~hit;
-      while (maybe != 0ull) {
-        // Exact test of this iteration's 128 original B points (the
-        // reference's formula, strict '<').
-        const uint32_t sl = (uint32_t)__builtin_ctzll(maybe);
-        maybe &= maybe - 1ull;
-        const float ax = ReadLaneF(m_ax, sl), ay = ReadLaneF(m_ay, sl);
-        float d0, d1;
-        if constexpr (ND == 2) {
-          d0 = D2(ax, ay, ocur[0][0], ocur[0][1]);
-          d1 = D2(ax, ay, ocur[1][0], ocur[1][1]);
-        } else {
-          const float az = ReadLaneF(m_az, sl);
-          d0 = D2(ax, ay, az, ocur[0][0], ocur[0][1], ocur[0][2]);
-          d1 = D2(ax, ay, az, ocur[1][0], ocur[1][1], ocur[1][2]);
+      if (maybe != 0ull) {
+        // Exact test of this iteration's 64 kFilterSteps original B points
+        // (the reference's formula, strict '<'), loaded only when a test is
+        // due.
+        float ob[kFilterSteps][ND];
+        uint32_t st = cur_step;
+  #pragma unroll
+        for (int k = 0; k < kFilterSteps; ++k) {
+          LoadStep<ND>(args.bx, args.by, args.bz, st, lane, ob[k]);
+          st = next_step(st);
         }
-        if (WaveAny(d0 < cmax2 || d1 < cmax2)) hit |= 1ull << sl;
+        while (maybe != 0ull) {
+          const uint32_t sl = (uint32_t)__builtin_ctzll(maybe);
+          maybe &= maybe - 1ull;
+          const float ax = ReadLaneF(m_ax, sl), ay = ReadLaneF(m_ay, sl);
+          bool below = false;
+          if constexpr (ND == 2) {
+  #pragma unroll
+            for (int k = 0; k < kFilterSteps; ++k)
+              below = below || D2(ax, ay, ob[k][0], ob[k][1]) < cmax2;
+          } else {
+            const float az = ReadLaneF(m_az, sl);
+  #pragma unroll
+            for (int k = 0; k < kFilterSteps; ++k)
+              below = below || D2(ax, ay, az, ob[k][0], ob[k][1], ob[k][2]) < cmax2;
+          }
+          if (WaveAny(below)) hit |= 1ull << sl;
+        }
       }
       if (split && hit != 0ull) {
         // Tell the other segments' waves this entry is decided.
@@ -967,13 +984,7 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
       }
       live &= ~(hit | gone | due);
   #pragma unroll
-      for (int q = 0; q < 8; ++q) bcur[q] = bnx[q];
-  #pragma unroll
-      for (int d = 0; d < ND; ++d) {
-        ocur[0][d] = onx[0][d];
-        ocur[1][d] = onx[1][d];
-      }
-      step = s2;
+      for (int q = 0; q < 4 * kFilterSteps; ++q) bcur[q] = bnx[q];
     }
   }
   if (n_stg > 0) flush();
