@@ -571,8 +571,7 @@ int hd_directed_device(hd_context* ctx, int variant, int n_dims,
   ScopedContext sc;
   HD_TRY(AcquireContext(ctx, &sc));
   float mbr_a[6], mbr_b[6];
-  HD_TRY(HostMbr(sc.ctx, n_dims, d_a, n_a, mbr_a));
-  HD_TRY(HostMbr(sc.ctx, n_dims, d_b, n_b, mbr_b));
+  HD_TRY(HostMbrPair(sc.ctx, n_dims, d_a, n_a, mbr_a, d_b, n_b, mbr_b));
   DirectedArgs args;
   args.n_dims = n_dims;
   args.a = d_a;
@@ -634,8 +633,7 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
     d_b = ub;
   }
   float mbr_a[6], mbr_b[6];
-  HD_TRY(HostMbr(c, nd, d_a, n_a, mbr_a));
-  HD_TRY(HostMbr(c, nd, d_b, n_b, mbr_b));
+  HD_TRY(HostMbrPair(c, nd, d_a, n_a, mbr_a, d_b, n_b, mbr_b));
 
   DirectedArgs ab;
   ab.n_dims = nd;

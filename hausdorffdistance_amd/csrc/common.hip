@@ -59,9 +59,39 @@ HD_DEV uint32_t OrderedBits(float f) {
 
 // Per block: min/max per coordinate and a count of non-finite coordinates.
 // out[0..ND) = ordered min, out[ND..2ND) = ordered max, out[2ND] = bad count.
+// The AoS array is read as flat floats, 16 bytes per load when it is 16-byte
+// aligned (value k of the array is coordinate k mod ND), four loads in
+// flight per thread.
+template <int ND>
+HD_DEV void MbrTake(float v, int d, uint32_t (&lo)[ND], uint32_t (&hi)[ND],
+                    uint32_t& bad) {
+  if (!isfinite(v)) {
+    bad++;
+    return;
+  }
+  const uint32_t k = OrderedBits(v);
+#pragma unroll
+  for (int e = 0; e < ND; ++e)
+    if (e == d) {
+      lo[e] = min(lo[e], k);
+      hi[e] = max(hi[e], k);
+    }
+}
+
+template <int ND>
+HD_DEV void MbrTake4(float4 v, uint64_t j, uint32_t (&lo)[ND],
+                     uint32_t (&hi)[ND], uint32_t& bad) {
+  // first coordinate index of the 4 values: 4 j mod ND
+  const int d0 = ND == 2 ? 0 : (int)((4ull * j) % 3ull);
+  MbrTake<ND>(v.x, d0, lo, hi, bad);
+  MbrTake<ND>(v.y, (d0 + 1) % ND, lo, hi, bad);
+  MbrTake<ND>(v.z, (d0 + 2) % ND, lo, hi, bad);
+  MbrTake<ND>(v.w, (d0 + 3) % ND, lo, hi, bad);
+}
+
 template <int ND>
 __global__ __launch_bounds__(256) void MbrKernel(const float* __restrict__ aos,
-                                                 uint64_t n,
+                                                 uint64_t n, int aligned,
                                                  uint32_t* __restrict__ out) {
   uint32_t lo[ND], hi[ND];
   uint32_t bad = 0;
@@ -70,20 +100,22 @@ __global__ __launch_bounds__(256) void MbrKernel(const float* __restrict__ aos,
     lo[d] = 0xFFFFFFFFu;
     hi[d] = 0u;
   }
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
+  const uint64_t total = n * ND;
+  const uint64_t n4 = aligned ? total / 4 : 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const float4* a4 = reinterpret_cast<const float4*>(aos);
+  uint64_t j = gid;
+  for (; j + 3 * stride < n4; j += 4 * stride) {
+    float4 v[4];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      float v = aos[i * ND + d];
-      if (!isfinite(v)) {
-        bad++;
-        continue;
-      }
-      uint32_t k = OrderedBits(v);
-      lo[d] = min(lo[d], k);
-      hi[d] = max(hi[d], k);
-    }
+    for (int u = 0; u < 4; ++u) v[u] = a4[j + u * stride];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) MbrTake4<ND>(v[u], j + u * stride, lo, hi, bad);
   }
+  for (; j < n4; j += stride) MbrTake4<ND>(a4[j], j, lo, hi, bad);
+  for (uint64_t k = 4 * n4 + gid; k < total; k += stride)
+    MbrTake<ND>(aos[k], (int)(k % ND), lo, hi, bad);
   __shared__ uint32_t s_lo[4][ND], s_hi[4][ND], s_bad[4];
   const int wave = threadIdx.x >> 6;
 #pragma unroll
@@ -234,32 +266,65 @@ int GatherSoA(hd_context* ctx, int n_dims, const float* aos, uint32_t n,
   return HD_OK;
 }
 
-int HostMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
-            float* mbr, uint32_t* d_work) {
-  if (n == 0) return Fail(HD_EINVAL, "empty point set");
-  uint32_t* d_out = d_work;
-  if (d_out == nullptr) HD_TRY(ContextArray(ctx, "mbr_keys", 8, &d_out));
-  // layout: lo[0..nd), hi[nd..2nd), non-finite count at [2nd]
+namespace {
+
+// Queues the MBR kernel of one set into d_out (2 n_dims + 1 words: ordered
+// lo, ordered hi, non-finite count).
+int LaunchMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
+              uint32_t* d_out) {
   HD_TRY(FillU32(ctx, d_out, 0xFFFFFFFFu, n_dims));
   HD_TRY(FillU32(ctx, d_out + n_dims, 0u, n_dims + 1));
-  const int grid = GridFor(n, 256, ctx->cu_count * 4);
+  const int grid = GridFor(n * n_dims / 4 + 1, 256, ctx->cu_count * 8);
+  const int aligned = (reinterpret_cast<uintptr_t>(aos) & 15u) == 0;
   if (n_dims == 2)
-    MbrKernel<2><<<grid, 256, 0, ctx->stream>>>(aos, n, d_out);
+    MbrKernel<2><<<grid, 256, 0, ctx->stream>>>(aos, n, aligned, d_out);
   else
-    MbrKernel<3><<<grid, 256, 0, ctx->stream>>>(aos, n, d_out);
+    MbrKernel<3><<<grid, 256, 0, ctx->stream>>>(aos, n, aligned, d_out);
   HD_HIP_TRY(hipGetLastError());
-  uint32_t h[8];
-  HD_HIP_TRY(hipMemcpyAsync(h, d_out, sizeof(uint32_t) * (2 * n_dims + 1),
-                            hipMemcpyDeviceToHost, ctx->stream));
-  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return HD_OK;
+}
+
+// Host MBR from the words LaunchMbr wrote; the reference's ToNonemptyMBR
+// widening (src/geoms/mbr.h:288-296).
+int DecodeMbr(int n_dims, const uint32_t* h, float* mbr) {
   if (h[2 * n_dims] != 0)
-    return Fail(HD_EINVAL, "%u non-finite coordinates in input",
-                h[2 * n_dims]);
+    return Fail(HD_EINVAL, "%u non-finite coordinates in input", h[2 * n_dims]);
   for (int d = 0; d < 2 * n_dims; ++d) mbr[d] = FromOrderedBits(h[d]);
   for (int d = 0; d < n_dims; ++d)
     if (mbr[n_dims + d] - mbr[d] == 0.0f)
       mbr[n_dims + d] = (float)((double)mbr[d] + 1e-5);  // mbr.h:288-296
   return HD_OK;
+}
+
+}  // namespace
+
+int HostMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
+            float* mbr, uint32_t* d_work) {
+  if (n == 0) return Fail(HD_EINVAL, "empty point set");
+  uint32_t* d_out = d_work;
+  if (d_out == nullptr) HD_TRY(ContextArray(ctx, "mbr_keys", 8, &d_out));
+  HD_TRY(LaunchMbr(ctx, n_dims, aos, n, d_out));
+  uint32_t h[8];
+  HD_HIP_TRY(hipMemcpyAsync(h, d_out, sizeof(uint32_t) * (2 * n_dims + 1),
+                            hipMemcpyDeviceToHost, ctx->stream));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return DecodeMbr(n_dims, h, mbr);
+}
+
+int HostMbrPair(hd_context* ctx, int n_dims, const float* a, uint64_t n_a,
+                float* mbr_a, const float* b, uint64_t n_b, float* mbr_b,
+                uint32_t* d_work) {
+  if (n_a == 0 || n_b == 0) return Fail(HD_EINVAL, "empty point set");
+  uint32_t* d_out = d_work;
+  if (d_out == nullptr) HD_TRY(ContextArray(ctx, "mbr_keys", 16, &d_out));
+  HD_TRY(LaunchMbr(ctx, n_dims, a, n_a, d_out));
+  HD_TRY(LaunchMbr(ctx, n_dims, b, n_b, d_out + 8));
+  uint32_t h[16];
+  HD_HIP_TRY(hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost,
+                            ctx->stream));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HD_TRY(DecodeMbr(n_dims, h, mbr_a));
+  return DecodeMbr(n_dims, h + 8, mbr_b);
 }
 
 int LaunchNnIndex(hd_context* ctx, int n_dims, const float* a,

@@ -376,10 +376,13 @@ __global__ __launch_bounds__(kBlock) void GridQueryKernel(GridQueryArgs args) {
 }  // namespace
 
 int BuildGrid(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
-              uint32_t n_points_cell, GridIndex* out) {
+              uint32_t n_points_cell, GridIndex* out, const float* mbr_b) {
   const int nd = n_dims;
   float mbr[6];
-  HD_TRY(HostMbr(ctx, nd, b, n_b, mbr));
+  if (mbr_b)
+    std::memcpy(mbr, mbr_b, sizeof(float) * 2 * nd);
+  else
+    HD_TRY(HostMbr(ctx, nd, b, n_b, mbr));
   GridDev& g = out->dev;
   std::memset(&g, 0, sizeof(g));
   // Resolution: reference uniform_grid.h:222-244 (cell side from the
@@ -516,7 +519,8 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
   const uint32_t n_a = (uint32_t)in.n_a;
   const int st_build = StageBegin(ctx, "grid_build");
   GridIndex grid;
-  HD_TRY(BuildGrid(ctx, ND, in.b, in.n_b, in.n_points_cell, &grid));
+  HD_TRY(BuildGrid(ctx, ND, in.b, in.n_b, in.n_points_cell, &grid,
+                   in.has_mbr ? in.mbr_b : nullptr));
   StageEnd(ctx, st_build);
 
   // key, seed and query counters, then the early-break counters of the

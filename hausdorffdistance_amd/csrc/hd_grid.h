@@ -55,8 +55,10 @@ struct GridQueryArgs {
   uint32_t skip_n;         // are skipped (done by a strided pass)
 };
 
+// mbr_b: B's MBR when the caller has it (host, 2 n_dims floats), else null.
 int BuildGrid(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
-              uint32_t n_points_cell, GridIndex* out);
+              uint32_t n_points_cell, GridIndex* out,
+              const float* mbr_b = nullptr);
 int LaunchGridQuery(hd_context* ctx, const GridQueryArgs& args);
 int GridNearestNeighbors(hd_context* ctx, int n_dims, const float* q,
                          uint64_t n_q, const float* ref, uint64_t n_ref,

@@ -67,8 +67,9 @@ struct LbvhQueryArgs {
   uint32_t skip_n;             // are skipped (done by a strided pass)
 };
 
+// mbr_b: B's MBR when the caller has it (host, 2 n_dims floats), else null.
 int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
-              uint32_t leaf_size, LbvhIndex* out);
+              uint32_t leaf_size, LbvhIndex* out, const float* mbr_b = nullptr);
 int LaunchLbvhQuery(hd_context* ctx, const LbvhQueryArgs& args);
 // Permutation of pts sorted by the top 32 bits of their Morton codes in
 // frame f (device array owned by the context).

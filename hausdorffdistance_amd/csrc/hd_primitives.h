@@ -50,6 +50,11 @@ int GatherSoA(hd_context* ctx, int n_dims, const float* aos, uint32_t n,
 // buffer.
 int HostMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
             float* mbr /* host, 2*n_dims */, uint32_t* d_work = nullptr);
+// Both sets' MBRs with one synchronisation. d_work: 16 device words, or null
+// for a context buffer.
+int HostMbrPair(hd_context* ctx, int n_dims, const float* a, uint64_t n_a,
+                float* mbr_a, const float* b, uint64_t n_b, float* mbr_b,
+                uint32_t* d_work = nullptr);
 
 }  // namespace hd
 

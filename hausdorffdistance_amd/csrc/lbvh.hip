@@ -496,12 +496,15 @@ int StartLeaves(hd_context* ctx, const LbvhIndex& tree, const float* q,
 }
 
 int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
-              uint32_t leaf_size, LbvhIndex* out) {
+              uint32_t leaf_size, LbvhIndex* out, const float* mbr_b) {
   const int nd = n_dims;
   const uint32_t n = (uint32_t)n_b;
   const uint32_t leaf = leaf_size ? leaf_size : 8u;
   float mbr[6];
-  HD_TRY(HostMbr(ctx, nd, b, n_b, mbr));
+  if (mbr_b)
+    std::memcpy(mbr, mbr_b, sizeof(float) * 2 * nd);
+  else
+    HD_TRY(HostMbr(ctx, nd, b, n_b, mbr));
   LbvhFrame f;
   std::memset(&f, 0, sizeof(f));
   // Largest quantisation level, exactly representable in FP32 so the
@@ -653,7 +656,8 @@ int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
   const uint32_t n_a = (uint32_t)in.n_a;
   const int st_build = StageBegin(ctx, "lbvh_build");
   LbvhIndex tree;
-  HD_TRY(BuildLbvh(ctx, ND, in.b, in.n_b, 8, &tree));
+  HD_TRY(BuildLbvh(ctx, ND, in.b, in.n_b, 8, &tree,
+                   in.has_mbr ? in.mbr_b : nullptr));
   StageEnd(ctx, st_build);
 
   unsigned long long* key = nullptr;

@@ -133,8 +133,7 @@ int SampledEbIterations(hd_context* ctx, const DirectedArgs& args,
     std::memcpy(mbr_a, args.mbr_a, sizeof(mbr_a));
     std::memcpy(mbr_b, args.mbr_b, sizeof(mbr_b));
   } else {
-    HD_TRY(HostMbr(ctx, nd, args.a, n_a, mbr_a));
-    HD_TRY(HostMbr(ctx, nd, args.b, n_b, mbr_b));
+    HD_TRY(HostMbrPair(ctx, nd, args.a, n_a, mbr_a, args.b, n_b, mbr_b));
   }
   float lo_a[3], hi_a[3], lo_b[3], hi_b[3], lb2 = 0.f, ub2 = 0.f;
   NonEmptyMbr(nd, mbr_a, lo_a, hi_a);
