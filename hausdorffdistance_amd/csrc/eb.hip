@@ -1129,6 +1129,179 @@ int LaunchSlots(hd_context* ctx, const float* a, int order_mode,
   return LaunchScan<ND>(ctx, args, pos_begin, pos_end - pos_begin);
 }
 
+// Short exact rounds (budget <= kDenseSteps steps) run without slots: every
+// point of the round is compared with one window of B, the budget's steps
+// from the launch's start step (where the slot kernel's waves start; a
+// point the slot kernel refills mid-way scans the steps after its refill
+// instead). The round is then a dense product of its points against the
+// window. Each wave holds the window in registers (lane l: point l of every
+// window step) and takes the round's positions 8 at a time as wave-uniform
+// values: 6 vector operations per evaluation and one cross-lane minimum per
+// point. A point is dropped (its window minimum is below cmax2) or
+// escalated with that exact partial minimum; escalations are staged in
+// lanes and appended 64 at a time. Drops and escalated minima are exact for
+// any window; the evaluations after a point's first hit are made too, and
+// counted.
+constexpr uint32_t kDenseSteps = 16u;
+constexpr int kDenseGroup = 8;  // positions per pass
+
+struct DenseArgs {
+  const float* a;
+  const uint32_t* order;
+  const float* list_min;
+  Feistel perm_a;
+  int order_mode;
+  uint32_t pos_begin, pos_end;
+  const float* bx;
+  const float* by;
+  const float* bz;
+  uint32_t n_steps;
+  uint32_t start_step;
+  uint32_t window_steps;
+  const unsigned long long* key;
+  uint32_t* esc_idx;
+  float* esc_min;
+  unsigned int* esc_count;
+  unsigned long long* evals;
+};
+
+template <int ND>
+__global__ __launch_bounds__(256) void EbDenseRoundKernel(DenseArgs args) {
+  const uint32_t lane = LaneId();
+  const uint32_t wave = Uniform((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const uint32_t n_waves = gridDim.x * (blockDim.x >> 6);
+  const uint32_t ws = args.window_steps;
+  float wb[kDenseSteps][ND];
+#pragma unroll
+  for (uint32_t k = 0; k < kDenseSteps; ++k) {
+    uint32_t st = args.start_step + (k < ws ? k : 0u);
+    st = st >= args.n_steps ? st - args.n_steps : st;
+    LoadStep<ND>(args.bx, args.by, args.bz, st, lane, wb[k]);
+  }
+  const uint32_t cm = __float_as_uint(LoadCmax2(args.key));
+  unsigned long long evals = 0;
+  uint32_t stg_idx = 0, n_stg = 0;
+  float stg_min = 0.f;
+  auto flush = [&]() __attribute__((always_inline)) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(args.esc_count, n_stg);
+    base = Uniform(base);
+    if (lane < n_stg) {
+      args.esc_idx[base + lane] = stg_idx;
+      args.esc_min[base + lane] = stg_min;
+    }
+    n_stg = 0;
+  };
+  const uint32_t n_pos = args.pos_end - args.pos_begin;
+  const uint32_t n_groups = (n_pos + kDenseGroup - 1) / kDenseGroup;
+  for (uint32_t g = wave; g < n_groups; g += n_waves) {
+    const uint32_t p0 = args.pos_begin + g * kDenseGroup;
+    // Lanes 0..7 fetch the group's points; the rest of the wave reads them
+    // as uniform values.
+    const uint32_t pos = p0 + (lane & (kDenseGroup - 1));
+    bool valid = lane < (uint32_t)kDenseGroup && pos < args.pos_end;
+    if (valid && args.list_min != nullptr)
+      valid = !(__float_as_uint(args.list_min[pos]) < cm);
+    uint32_t idx = 0;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (valid) {
+      idx = OrderAt(args.order_mode, args.order, args.perm_a, pos);
+      const float* src = args.a + (uint64_t)idx * ND;
+      px = src[0];
+      py = src[1];
+      if constexpr (ND == 3) pz = src[2];
+    }
+    const uint32_t vm = (uint32_t)__ballot(valid);
+    uint32_t mine = kFltMaxBits;  // lane p < 8: point p's window minimum
+#pragma unroll
+    for (int q = 0; q < kDenseGroup; ++q) {
+      if (!((vm >> q) & 1u)) continue;
+      const float ax = ReadLaneF(px, (uint32_t)q), ay = ReadLaneF(py, (uint32_t)q);
+      const float az = ND == 3 ? ReadLaneF(pz, (uint32_t)q) : 0.f;
+      uint32_t m = kFltMaxBits;
+#pragma unroll
+      for (uint32_t k = 0; k < kDenseSteps; ++k) {
+        if (k >= ws) break;
+        float d;
+        if constexpr (ND == 2)
+          d = D2(ax, ay, wb[k][0], wb[k][1]);
+        else
+          d = D2(ax, ay, az, wb[k][0], wb[k][1], wb[k][2]);
+        m = MinBits(m, __float_as_uint(d));
+      }
+      m = WaveMinU32(m);
+      if (lane == (uint32_t)q) mine = m;
+    }
+    evals += (unsigned long long)__popc(vm) * (64ull * ws);
+    // Escalations into lanes [n_stg, n_stg + n_esc), appended 64 at a time.
+    const bool esc = valid && !(mine < cm);
+    const unsigned long long em = __ballot(esc);
+    const uint32_t n_esc = (uint32_t)__popcll(em);
+    if (n_esc == 0u) continue;
+    if (n_stg + n_esc > 64u) flush();
+    // A forward permutation: escalating lanes to [n_stg, n_stg + n_esc) in
+    // order, the other lanes after them.
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(em >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)em, 0u));
+    const uint32_t dest = esc ? n_stg + below : n_stg + n_esc + (lane - below);
+    const int d4 = (int)((dest & 63u) << 2);
+    const uint32_t p_idx = (uint32_t)__builtin_amdgcn_ds_permute(d4, esc ? (int)idx : 0);
+    const uint32_t p_min = (uint32_t)__builtin_amdgcn_ds_permute(d4, esc ? (int)mine : 0);
+    if (lane >= n_stg && lane < n_stg + n_esc) {
+      stg_idx = p_idx;
+      stg_min = __uint_as_float(p_min);
+    }
+    n_stg += n_esc;
+    if (n_stg == 64u) flush();
+  }
+  if (n_stg > 0u) flush();
+  if (lane == 0) atomicAdd(args.evals, evals);
+}
+
+template <int ND>
+int LaunchDenseRound(hd_context* ctx, const float* a, int order_mode,
+                     const uint32_t* order, const float* list_min,
+                     const Feistel& perm_a, uint32_t pos_begin,
+                     uint32_t pos_end, const float* bx, const float* by,
+                     const float* bz, uint32_t n_steps, uint32_t budget_steps,
+                     unsigned long long* key, unsigned long long* evals,
+                     unsigned int* counters, uint32_t* esc_idx, float* esc_min,
+                     bool reset_esc) {
+  if (pos_begin >= pos_end) return HD_OK;
+  if (budget_steps == 0 || budget_steps > kDenseSteps || budget_steps >= n_steps)
+    return Fail(HD_EINVAL, "dense round: budget %u steps", budget_steps);
+  if (reset_esc) HD_TRY(FillU32(ctx, counters + 1, 0u, 1));
+  DenseArgs args = {};
+  args.a = a;
+  args.order = order;
+  args.list_min = list_min;
+  args.perm_a = perm_a;
+  args.order_mode = order_mode;
+  args.pos_begin = pos_begin;
+  args.pos_end = pos_end;
+  args.bx = bx;
+  args.by = by;
+  args.bz = bz;
+  args.n_steps = n_steps;
+  // The window the slot kernel's waves would scan: from the launch's start
+  // step, budget_steps steps, wrapping at the end of B.
+  args.start_step = RoundStartStep(pos_begin, budget_steps, n_steps);
+  args.window_steps = budget_steps;
+  args.key = key;
+  args.esc_idx = esc_idx;
+  args.esc_min = esc_min;
+  args.esc_count = counters + 1;
+  args.evals = evals;
+  const uint64_t groups = (pos_end - pos_begin + kDenseGroup - 1) / kDenseGroup;
+  // Enough waves for every SIMD, each with at least 16 groups.
+  const int blocks = (int)std::max<uint64_t>(
+      1, std::min<uint64_t>((groups + 63u) / 64u, (uint64_t)ctx->cu_count * 8u));
+  StageDispatch(ctx);
+  EbDenseRoundKernel<ND><<<blocks, 256, 0, ctx->stream>>>(args);
+  HD_HIP_TRY(hipGetLastError());
+  return HD_OK;
+}
+
 // Centred B tiles for the matrix-core filter: tile q holds points
 // 16q..16q+15 of the shuffled SoA copy as [x' x16, y' x16, |b'|^2 x16, 1 x16]
 // (2D) or [x', y', z', |b'|^2 + C] (3D).
@@ -1540,6 +1713,10 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
       if (filt)
         return LaunchMfma<ND>(ctx, a, m, l, lm, perm_a, b, e, t, budget, key,
                               evc, counters, idx_out, min_out, reset);
+      if (budget <= kDenseSteps && budget < t.n_steps)
+        return LaunchDenseRound<ND>(ctx, a, m, l, lm, perm_a, b, e, t.bx, t.by,
+                                    t.bz, t.n_steps, budget, key, evc,
+                                    counters, idx_out, min_out, reset);
       return LaunchSlots<ND>(ctx, a, m, l, lm, perm_a, b, e, t.bx, t.by, t.bz,
                              t.n_steps, budget, key, evc, counters, idx_out,
                              min_out, reset);
