@@ -138,7 +138,7 @@ SIGNATURES = {
     "hd_filter_probe_device": (
         C.c_int, [_P, C.c_int, _P, C.c_uint64, _P, C.c_uint64, _P, _P, _F]),
     "hd_sort_pairs_probe_device": (
-        C.c_int, [_P, C.c_int, _P, _P, C.c_uint32, C.c_int, _P, _P]),
+        C.c_int, [_P, C.c_int, _P, _P, C.c_uint32, C.c_int, C.c_int, _P, _P]),
     "hd_directed_host": (
         C.c_int,
         [C.c_int, _P, C.c_uint64, _P, C.c_uint64, C.c_uint32, C.c_int32, C.c_int32,

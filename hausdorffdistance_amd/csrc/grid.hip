@@ -577,7 +577,12 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
       f.scale[d] = (float)(levels / ((double)grid.dev.hi[d] - (double)grid.dev.lo[d]));
     }
     uint32_t* mo = nullptr;
-    HD_TRY(MortonOrder(ctx, ND, in.a, n_a, f, &mo));
+    // The grid's queries need locality, not the full code: the top 16 bits
+    // (a 256 x 256 grid over the frame in 2D) in two radix passes instead of
+    // four (10M GMM: grid 2.63 -> 2.51 ms). The LBVH keeps all 32 bits; its
+    // traversals lose more than the sort saves with the coarse order
+    // (2.97 -> 3.78 ms).
+    HD_TRY(MortonOrder(ctx, ND, in.a, n_a, f, &mo, 16));
     StageEnd(ctx, st_ord);
     qa.order = mo;
     qa.order_mode = 0;

@@ -71,10 +71,11 @@ struct LbvhQueryArgs {
 int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
               uint32_t leaf_size, LbvhIndex* out, const float* mbr_b = nullptr);
 int LaunchLbvhQuery(hd_context* ctx, const LbvhQueryArgs& args);
-// Permutation of pts sorted by the top 32 bits of their Morton codes in
-// frame f (device array owned by the context).
+// Permutation of pts sorted by bits [begin_bit, 32) of their 32-bit Morton
+// codes in frame f, equal keys in input order (device array owned by the
+// context).
 int MortonOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
-                const LbvhFrame& f, uint32_t** order_out);
+                const LbvhFrame& f, uint32_t** order_out, int begin_bit = 0);
 // start[p] = leaf of tree holding the first B code >= the Morton code of
 // query q[order[p]] (clamped to the last leaf); order should be a Morton
 // order of the queries, for coherent searches.

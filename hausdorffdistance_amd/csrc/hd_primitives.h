@@ -10,15 +10,16 @@
 namespace hd {
 
 // Stable radix sorts of (key, u32 value) pairs (sort.hip): ascending on key
-// bits [0, end_bit) for u32, on all 16 bits for u16, descending for finite
-// f32 keys. Inputs and outputs must not alias.
+// bits [begin_bit, end_bit) for u32 (the other bits are carried), on all 16
+// bits for u16, descending for finite f32 keys. Inputs and outputs must not
+// alias.
 int SortPairsDescF32(hd_context* ctx, const float* k_in, float* k_out,
                      const uint32_t* v_in, uint32_t* v_out, uint32_t n);
 int SortPairsU16(hd_context* ctx, const uint16_t* k_in, uint16_t* k_out,
                  const uint32_t* v_in, uint32_t* v_out, uint32_t n);
 int SortPairsU32(hd_context* ctx, const uint32_t* k_in, uint32_t* k_out,
                  const uint32_t* v_in, uint32_t* v_out, uint32_t n,
-                 int end_bit);
+                 int end_bit, int begin_bit = 0);
 
 // Host construction of a Feistel permutation of [0, n) (hd_device.h).
 Feistel MakeFeistel(uint32_t n, uint64_t seed, uint32_t stream_id);

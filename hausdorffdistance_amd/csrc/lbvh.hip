@@ -461,7 +461,7 @@ __global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
 }  // namespace
 
 int MortonOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
-                const LbvhFrame& f, uint32_t** order_out) {
+                const LbvhFrame& f, uint32_t** order_out, int begin_bit) {
   uint32_t *keys = nullptr, *keys_s = nullptr;
   uint32_t *vals = nullptr, *order = nullptr;
   HD_TRY(ContextArray(ctx, "lbvh_qkeys", n, &keys));
@@ -474,7 +474,7 @@ int MortonOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
   else
     MortonKernel32<3><<<blocks, 256, 0, ctx->stream>>>(pts, n, f, keys, vals);
   HD_HIP_TRY(hipGetLastError());
-  HD_TRY(SortPairsU32(ctx, keys, keys_s, vals, order, n, 32));
+  HD_TRY(SortPairsU32(ctx, keys, keys_s, vals, order, n, 32, begin_bit));
   *order_out = order;
   return HD_OK;
 }

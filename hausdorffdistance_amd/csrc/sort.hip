@@ -237,12 +237,14 @@ template <class Codec>
 int RadixSortPairs(hd_context* ctx, const char* what,
                    const typename Codec::Stored* k_in,
                    typename Codec::Stored* k_out, const uint32_t* v_in,
-                   uint32_t* v_out, uint32_t n, int end_bit) {
+                   uint32_t* v_out, uint32_t n, int end_bit, int begin_bit = 0) {
   using Stored = typename Codec::Stored;
   if (n == 0) return HD_OK;
-  if (end_bit < 0 || end_bit > 32 || (k_in == k_out) || (v_in == v_out))
-    return Fail(HD_EINVAL, "%s: bad arguments (end_bit %d)", what, end_bit);
-  const int passes = std::max(1, (end_bit + 7) / 8);
+  if (end_bit < 0 || end_bit > 32 || begin_bit < 0 || begin_bit > end_bit ||
+      (k_in == k_out) || (v_in == v_out))
+    return Fail(HD_EINVAL, "%s: bad arguments (bits %d..%d)", what, begin_bit,
+                end_bit);
+  const int passes = std::max(1, (end_bit - begin_bit + 7) / 8);
   const uint32_t n_tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
   uint32_t *counts = nullptr, *totals = nullptr;
   HD_TRY(ContextArray(ctx, "sort_counts", (size_t)kRadix * n_tiles, &counts));
@@ -260,8 +262,9 @@ int RadixSortPairs(hd_context* ctx, const char* what,
     const bool to_out = ((passes - 1 - p) & 1) == 0;
     Stored* dst_k = to_out ? k_out : tmp_k;
     uint32_t* dst_v = to_out ? v_out : tmp_v;
-    const int shift = 8 * p;
-    const int bits = std::min(8, end_bit - shift);
+    const int bits = std::min(8, end_bit - (begin_bit + 8 * p));
+    // An empty bit range is one stable copy (digit 0 for every key).
+    const int shift = bits > 0 ? begin_bit + 8 * p : 0;
     const uint32_t mask = bits > 0 ? (1u << bits) - 1u : 0u;
     RadixCountKernel<Codec><<<n_tiles, kBlock, 0, ctx->stream>>>(
         src_k, n, shift, mask, n_tiles, counts);
@@ -292,27 +295,28 @@ int SortPairsU16(hd_context* ctx, const uint16_t* k_in, uint16_t* k_out,
 
 int SortPairsU32(hd_context* ctx, const uint32_t* k_in, uint32_t* k_out,
                  const uint32_t* v_in, uint32_t* v_out, uint32_t n,
-                 int end_bit) {
+                 int end_bit, int begin_bit) {
   return RadixSortPairs<CodecU32>(ctx, "sort<u32>", k_in, k_out, v_in, v_out,
-                                  n, end_bit);
+                                  n, end_bit, begin_bit);
 }
 
 }  // namespace hd
 
 // Test hook: sorts n (key, value) pairs on the context stream and waits.
-// key_kind 0: u16 keys (16 bits), 1: u32 keys on bits [0, end_bit),
+// key_kind 0: u16 keys (16 bits), 1: u32 keys on bits [begin_bit, end_bit),
 // 2: f32 keys, descending.
 extern "C" int hd_sort_pairs_probe_device(hd_context* ctx, int key_kind,
                                           const void* d_keys_in,
                                           const uint32_t* d_vals_in,
-                                          uint32_t n, int end_bit,
-                                          void* d_keys_out,
+                                          uint32_t n, int begin_bit,
+                                          int end_bit, void* d_keys_out,
                                           uint32_t* d_vals_out) {
   using namespace hd;
   ClearError();
   if (!ctx || (n && (!d_keys_in || !d_vals_in || !d_keys_out || !d_vals_out)))
     return Fail(HD_EINVAL, "null argument");
-  if (end_bit < 0 || end_bit > 32) return Fail(HD_EINVAL, "end_bit must be in [0, 32]");
+  if (end_bit < 0 || end_bit > 32 || begin_bit < 0 || begin_bit > end_bit)
+    return Fail(HD_EINVAL, "bits must satisfy 0 <= begin_bit <= end_bit <= 32");
   HD_TRY(ContextActivate(ctx));
   int rc;
   if (key_kind == 0)
@@ -322,7 +326,7 @@ extern "C" int hd_sort_pairs_probe_device(hd_context* ctx, int key_kind,
   else if (key_kind == 1)
     rc = SortPairsU32(ctx, static_cast<const uint32_t*>(d_keys_in),
                       static_cast<uint32_t*>(d_keys_out), d_vals_in,
-                      d_vals_out, n, end_bit);
+                      d_vals_out, n, end_bit, begin_bit);
   else if (key_kind == 2)
     rc = SortPairsDescF32(ctx, static_cast<const float*>(d_keys_in),
                           static_cast<float*>(d_keys_out), d_vals_in,

@@ -322,12 +322,13 @@ int hd_filter_probe_device(hd_context* ctx, int n_dims, const float* d_a,
  * (the reference sorts Morton codes with thrust::sort_by_key,
  * src/index/uniform_grid.h, src/lbvh/lbvh/bvh.cuh): sorts n device pairs on
  * the context stream and waits. key_kind 0: u16 keys, ascending; 1: u32
- * keys, ascending on bits [0, end_bit); 2: finite f32 keys, descending.
- * Equal keys keep their input order. Outputs must not alias inputs. */
+ * keys, ascending on bits [begin_bit, end_bit) (other bits carried);
+ * 2: finite f32 keys, descending. Equal keys keep their input order.
+ * Outputs must not alias inputs. */
 int hd_sort_pairs_probe_device(hd_context* ctx, int key_kind,
                                const void* d_keys_in, const uint32_t* d_vals_in,
-                               uint32_t n, int end_bit, void* d_keys_out,
-                               uint32_t* d_vals_out);
+                               uint32_t n, int begin_bit, int end_bit,
+                               void* d_keys_out, uint32_t* d_vals_out);
 
 /* Host-side pieces of the input statistics (stats_n_points_cell), as test
  * hooks: for the multiset whose value v occurs hist[v] times (the number of

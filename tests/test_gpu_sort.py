@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 U16, U32, F32_DESC = 0, 1, 2
 
 
-def device_sort(ctx, kind, keys: np.ndarray, vals: np.ndarray, end_bit: int):
+def device_sort(ctx, kind, keys: np.ndarray, vals: np.ndarray, end_bit: int, begin_bit: int = 0):
     import torch
 
     tk = torch.from_numpy(keys.view(np.int16 if keys.dtype == np.uint16 else np.int32)).cuda()
@@ -26,29 +26,30 @@ def device_sort(ctx, kind, keys: np.ndarray, vals: np.ndarray, end_bit: int):
     ok, ov = torch.empty_like(tk), torch.empty_like(tv)
     rc = _lib.load().hd_sort_pairs_probe_device(
         ctx.handle, kind, C.c_void_p(tk.data_ptr()), C.c_void_p(tv.data_ptr()), keys.size,
-        end_bit, C.c_void_p(ok.data_ptr()), C.c_void_p(ov.data_ptr()))
+        begin_bit, end_bit, C.c_void_p(ok.data_ptr()), C.c_void_p(ov.data_ptr()))
     assert rc == 0, _lib.load().hd_last_error().decode()
     return ok.cpu().numpy().view(keys.dtype), ov.cpu().numpy().view(np.uint32)
 
 
-def expected_order(kind, keys: np.ndarray, end_bit: int) -> np.ndarray:
+def expected_order(kind, keys: np.ndarray, end_bit: int, begin_bit: int = 0) -> np.ndarray:
     if kind == U16:
         return np.argsort(keys, kind="stable")
     if kind == U32:
-        mask = np.uint32((1 << end_bit) - 1) if end_bit < 32 else np.uint32(0xFFFFFFFF)
-        return np.argsort(keys & mask, kind="stable")
+        k = keys.astype(np.uint64)
+        mask = np.uint64((1 << (end_bit - begin_bit)) - 1)
+        return np.argsort((k >> np.uint64(begin_bit)) & mask, kind="stable")
     # descending IEEE total order (+0.0 before -0.0)
     u = keys.view(np.uint32)
     o = np.where(u & np.uint32(0x80000000), ~u, u | np.uint32(0x80000000))
     return np.argsort(~o, kind="stable")
 
 
-def check(ctx, kind, keys, end_bit=32, vals=None):
+def check(ctx, kind, keys, end_bit=32, vals=None, begin_bit=0):
     n = keys.size
     if vals is None:
         vals = np.arange(n, dtype=np.uint32)
-    ok, ov = device_sort(ctx, kind, keys, vals, end_bit)
-    order = expected_order(kind, keys, end_bit)
+    ok, ov = device_sort(ctx, kind, keys, vals, end_bit, begin_bit)
+    order = expected_order(kind, keys, end_bit, begin_bit)
     np.testing.assert_array_equal(ov, vals[order])
     np.testing.assert_array_equal(ok.view(np.uint32 if kind != U16 else np.uint16),
                                   keys[order].view(np.uint32 if kind != U16 else np.uint16))
@@ -76,6 +77,14 @@ def test_u32_low_bits_only(ctx, end_bit):
     """Bits at and above end_bit are carried, not sorted on."""
     rng = np.random.default_rng(end_bit)
     check(ctx, U32, rng.integers(0, 2**32, 50_000, dtype=np.uint32), end_bit)
+
+
+@pytest.mark.parametrize("bits", [(16, 32), (8, 20), (31, 32), (5, 5)])
+def test_u32_bit_range(ctx, bits):
+    """Bits [begin_bit, end_bit) only (the query-order sort uses 16..32)."""
+    begin, end = bits
+    rng = np.random.default_rng(begin * 100 + end)
+    check(ctx, U32, rng.integers(0, 2**32, 70_000, dtype=np.uint32), end, begin_bit=begin)
 
 
 def test_u32_presorted_and_reversed(ctx):
