@@ -135,6 +135,51 @@ class DirectedResult:
                               r.time_ms)
 
 
+class BatchResults:
+    """Per-pair results of a batch (a sequence of DirectedResult, built on
+    access) over one array of the C results; the fields are also available
+    as numpy arrays (cmax2_bits, argmax, nn, hd) for all pairs at once."""
+
+    _dtype = None
+
+    def __init__(self, out, n_pairs: int):
+        if BatchResults._dtype is None:
+            BatchResults._dtype = np.ctypeslib.as_array((_lib.DirectedResultC * 1)()).dtype
+        self._out = out  # keeps the ctypes buffer alive
+        self._rows = np.frombuffer(out, dtype=BatchResults._dtype, count=n_pairs)
+
+    def __len__(self) -> int:
+        return int(self._rows.shape[0])
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[k] for k in range(*i.indices(len(self)))]
+        r = self._rows[i]
+        return DirectedResult(int(r["cmax2_bits"]), int(r["argmax"]), int(r["nn"]),
+                              int(r["variant_used"]), float(r["hd"]), int(r["evals"]),
+                              float(r["time_ms"]))
+
+    def __iter__(self):
+        for i in range(len(self)):
+            yield self[i]
+
+    @property
+    def cmax2_bits(self) -> np.ndarray:
+        return self._rows["cmax2_bits"].copy()
+
+    @property
+    def argmax(self) -> np.ndarray:
+        return self._rows["argmax"].copy()
+
+    @property
+    def nn(self) -> np.ndarray:
+        return self._rows["nn"].copy()
+
+    @property
+    def hd(self) -> np.ndarray:
+        return self._rows["hd"].copy()
+
+
 @dataclass
 class Result:
     hd: float
@@ -402,10 +447,12 @@ class HausdorffDistanceNearestNeighborSearch(HausdorffDistance):
     algorithm = "Nearest Neighbor Search"
 
 
-def batch_directed(ctx: Context, points_a, offs_a, points_b, offs_b) -> list:
+def batch_directed(ctx: Context, points_a, offs_a, points_b, offs_b) -> "BatchResults":
     """Directed h(A_p -> B_p) for every pair p of a batch (CUDA tensors:
     points (N, d) float32, offsets (n_pairs + 1,) int64 in points). One
-    persistent-kernel launch replayed from a cached HIP graph."""
+    persistent-kernel launch replayed from a cached HIP graph. Returns a
+    BatchResults: result p is a DirectedResult, the fields of all pairs are
+    numpy arrays."""
     nd = int(points_a.shape[1])
     pa, n_pts_a, da, _ = _points(points_a, nd)
     pb, n_pts_b, db, _ = _points(points_b, nd)
@@ -429,7 +476,7 @@ def batch_directed(ctx: Context, points_a, offs_a, points_b, offs_b) -> list:
     _lib.check(_lib.load().hd_batch_directed_device(
         ctx.handle, nd, pa, C.c_void_p(offs_a.data_ptr()), pb, C.c_void_p(offs_b.data_ptr()),
         n_pairs, 0, out))
-    return [DirectedResult.from_c(out[i]) for i in range(n_pairs)]
+    return BatchResults(out, n_pairs)
 
 
 class Model:
