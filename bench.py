@@ -206,10 +206,6 @@ def eb_roofline(stages: dict, steps: int) -> dict:
             "frac": achieved / MFMA_F32_TFLOPS if achieved else None,
             "traffic": None, "algorithmic_flop": flop, "kernel_ms": t_kernel * 1e3,
             "dispatches_per_step": disp / max(1, steps),
-            # the streaming view of the same dispatches: 8 B of B per
-            # evaluated pair, as if no tile were reused (above 1: reuse)
-            "streaming_model_frac": (8.0 * evals / launches / t_kernel / 1e9 / HBM_PEAK_GBS)
-            if t_kernel > 0 else None,
             "kernels": other,
         }
     bytes_alg = 8.0 * (points + evals) / launches
