@@ -57,8 +57,11 @@ HD_DEV uint32_t OrderedBits(float f) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// Per block: min/max per coordinate and a count of non-finite coordinates.
-// out[0..ND) = ordered min, out[ND..2ND) = ordered max, out[2ND] = bad count.
+// Per block: min/max per coordinate and a count of non-finite coordinates,
+// folded into slot (block mod n_slots) of out, kMbrSlotWords words apart:
+// slot[0..ND) = ordered min, slot[ND..2ND) = ordered max, slot[2ND] = bad
+// count. Spreading the blocks' atomics over slots (cache lines) keeps them
+// from queueing on one address; the host folds the slots.
 // The AoS array is read as flat floats, 16 bytes per load when it is 16-byte
 // aligned (value k of the array is coordinate k mod ND), four loads in
 // flight per thread.
@@ -89,10 +92,22 @@ HD_DEV void MbrTake4(float4 v, uint64_t j, uint32_t (&lo)[ND],
   MbrTake<ND>(v.w, (d0 + 3) % ND, lo, hi, bad);
 }
 
+constexpr uint32_t kMbrSlotWords = 32;  // 128 bytes per slot
+constexpr uint32_t kMbrSlots = 32;
+
+__global__ void MbrInitKernel(uint32_t* out, int n_dims, uint32_t n_slots) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_slots * kMbrSlotWords) return;
+  const uint32_t w = i % kMbrSlotWords;
+  out[i] = w < (uint32_t)n_dims ? 0xFFFFFFFFu : 0u;
+}
+
 template <int ND>
 __global__ __launch_bounds__(256) void MbrKernel(const float* __restrict__ aos,
                                                  uint64_t n, int aligned,
-                                                 uint32_t* __restrict__ out) {
+                                                 uint32_t n_slots,
+                                                 uint32_t* __restrict__ out_all) {
+  uint32_t* out = out_all + (blockIdx.x % n_slots) * kMbrSlotWords;
   uint32_t lo[ND], hi[ND];
   uint32_t bad = 0;
 #pragma unroll
@@ -268,63 +283,89 @@ int GatherSoA(hd_context* ctx, int n_dims, const float* aos, uint32_t n,
 
 namespace {
 
-// Queues the MBR kernel of one set into d_out (2 n_dims + 1 words: ordered
-// lo, ordered hi, non-finite count).
+// Queues the MBR kernel of one set into d_out (n_slots slots of
+// kMbrSlotWords words).
 int LaunchMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
-              uint32_t* d_out) {
-  HD_TRY(FillU32(ctx, d_out, 0xFFFFFFFFu, n_dims));
-  HD_TRY(FillU32(ctx, d_out + n_dims, 0u, n_dims + 1));
+              uint32_t* d_out, uint32_t n_slots) {
+  MbrInitKernel<<<(n_slots * kMbrSlotWords + 255) / 256, 256, 0, ctx->stream>>>(
+      d_out, n_dims, n_slots);
   const int grid = GridFor(n * n_dims / 4 + 1, 256, ctx->cu_count * 8);
   const int aligned = (reinterpret_cast<uintptr_t>(aos) & 15u) == 0;
   if (n_dims == 2)
-    MbrKernel<2><<<grid, 256, 0, ctx->stream>>>(aos, n, aligned, d_out);
+    MbrKernel<2><<<grid, 256, 0, ctx->stream>>>(aos, n, aligned, n_slots, d_out);
   else
-    MbrKernel<3><<<grid, 256, 0, ctx->stream>>>(aos, n, aligned, d_out);
+    MbrKernel<3><<<grid, 256, 0, ctx->stream>>>(aos, n, aligned, n_slots, d_out);
   HD_HIP_TRY(hipGetLastError());
   return HD_OK;
 }
 
-// Host MBR from the words LaunchMbr wrote; the reference's ToNonemptyMBR
+// Host MBR from the slots LaunchMbr wrote; the reference's ToNonemptyMBR
 // widening (src/geoms/mbr.h:288-296).
-int DecodeMbr(int n_dims, const uint32_t* h, float* mbr) {
-  if (h[2 * n_dims] != 0)
-    return Fail(HD_EINVAL, "%u non-finite coordinates in input", h[2 * n_dims]);
-  for (int d = 0; d < 2 * n_dims; ++d) mbr[d] = FromOrderedBits(h[d]);
+int DecodeMbr(int n_dims, const uint32_t* h, uint32_t n_slots, float* mbr) {
+  uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0, 0, 0};
+  uint64_t bad = 0;
+  for (uint32_t k = 0; k < n_slots; ++k) {
+    const uint32_t* w = h + k * kMbrSlotWords;
+    for (int d = 0; d < n_dims; ++d) {
+      lo[d] = std::min(lo[d], w[d]);
+      hi[d] = std::max(hi[d], w[n_dims + d]);
+    }
+    bad += w[2 * n_dims];
+  }
+  if (bad != 0)
+    return Fail(HD_EINVAL, "%llu non-finite coordinates in input",
+                (unsigned long long)bad);
+  for (int d = 0; d < n_dims; ++d) {
+    mbr[d] = FromOrderedBits(lo[d]);
+    mbr[n_dims + d] = FromOrderedBits(hi[d]);
+  }
   for (int d = 0; d < n_dims; ++d)
     if (mbr[n_dims + d] - mbr[d] == 0.0f)
       mbr[n_dims + d] = (float)((double)mbr[d] + 1e-5);  // mbr.h:288-296
   return HD_OK;
 }
 
+// Slots per set: one in a caller-provided workspace (kMbrSlotWords words per
+// set, the allocation-free path), kMbrSlots in the context's buffer.
+uint32_t MbrSlots(const uint32_t* d_work) { return d_work ? 1u : kMbrSlots; }
+
 }  // namespace
 
 int HostMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
             float* mbr, uint32_t* d_work) {
   if (n == 0) return Fail(HD_EINVAL, "empty point set");
+  // A caller-provided workspace (the allocation-free path) holds one slot.
+  const uint32_t slots = MbrSlots(d_work);
   uint32_t* d_out = d_work;
-  if (d_out == nullptr) HD_TRY(ContextArray(ctx, "mbr_keys", 8, &d_out));
-  HD_TRY(LaunchMbr(ctx, n_dims, aos, n, d_out));
-  uint32_t h[8];
-  HD_HIP_TRY(hipMemcpyAsync(h, d_out, sizeof(uint32_t) * (2 * n_dims + 1),
+  if (d_out == nullptr)
+    HD_TRY(ContextArray(ctx, "mbr_keys", 2 * kMbrSlots * kMbrSlotWords, &d_out));
+  HD_TRY(LaunchMbr(ctx, n_dims, aos, n, d_out, slots));
+  // A host array on the stack: this path allocates nothing (it also serves
+  // the allocation-free entry point).
+  uint32_t h[kMbrSlots * kMbrSlotWords];
+  HD_HIP_TRY(hipMemcpyAsync(h, d_out, sizeof(uint32_t) * slots * kMbrSlotWords,
                             hipMemcpyDeviceToHost, ctx->stream));
   HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  return DecodeMbr(n_dims, h, mbr);
+  return DecodeMbr(n_dims, h, slots, mbr);
 }
 
 int HostMbrPair(hd_context* ctx, int n_dims, const float* a, uint64_t n_a,
                 float* mbr_a, const float* b, uint64_t n_b, float* mbr_b,
                 uint32_t* d_work) {
   if (n_a == 0 || n_b == 0) return Fail(HD_EINVAL, "empty point set");
+  const uint32_t slots = MbrSlots(d_work);
+  const uint32_t words = slots * kMbrSlotWords;
   uint32_t* d_out = d_work;
-  if (d_out == nullptr) HD_TRY(ContextArray(ctx, "mbr_keys", 16, &d_out));
-  HD_TRY(LaunchMbr(ctx, n_dims, a, n_a, d_out));
-  HD_TRY(LaunchMbr(ctx, n_dims, b, n_b, d_out + 8));
-  uint32_t h[16];
-  HD_HIP_TRY(hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost,
-                            ctx->stream));
+  if (d_out == nullptr)
+    HD_TRY(ContextArray(ctx, "mbr_keys", 2 * kMbrSlots * kMbrSlotWords, &d_out));
+  HD_TRY(LaunchMbr(ctx, n_dims, a, n_a, d_out, slots));
+  HD_TRY(LaunchMbr(ctx, n_dims, b, n_b, d_out + words, slots));
+  uint32_t h[2 * kMbrSlots * kMbrSlotWords];
+  HD_HIP_TRY(hipMemcpyAsync(h, d_out, sizeof(uint32_t) * 2 * words,
+                            hipMemcpyDeviceToHost, ctx->stream));
   HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  HD_TRY(DecodeMbr(n_dims, h, mbr_a));
-  return DecodeMbr(n_dims, h + 8, mbr_b);
+  HD_TRY(DecodeMbr(n_dims, h, slots, mbr_a));
+  return DecodeMbr(n_dims, h + words, slots, mbr_b);
 }
 
 int LaunchNnIndex(hd_context* ctx, int n_dims, const float* a,

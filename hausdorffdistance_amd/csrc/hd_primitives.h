@@ -47,12 +47,12 @@ int GatherSoA(hd_context* ctx, int n_dims, const float* aos, uint32_t n,
 
 // Reads the MBR back (synchronous) and applies the reference's
 // ToNonemptyMBR widening (src/geoms/mbr.h:288-296). Non-finite coordinates
-// are an HD_EINVAL error. d_work: 8 device words, or null for a context
+// are an HD_EINVAL error. d_work: 32 device words, or null for a context
 // buffer.
 int HostMbr(hd_context* ctx, int n_dims, const float* aos, uint64_t n,
             float* mbr /* host, 2*n_dims */, uint32_t* d_work = nullptr);
-// Both sets' MBRs with one synchronisation. d_work: 16 device words, or null
-// for a context buffer.
+// Both sets' MBRs with one synchronisation. d_work: 64 device words, or
+// null for a context buffer.
 int HostMbrPair(hd_context* ctx, int n_dims, const float* a, uint64_t n_a,
                 float* mbr_a, const float* b, uint64_t n_b, float* mbr_b,
                 uint32_t* d_work = nullptr);

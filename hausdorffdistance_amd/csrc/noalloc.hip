@@ -33,7 +33,7 @@ constexpr size_t kOffKey = 0;         // u64 key, u64 evaluation counter
 constexpr size_t kOffCounters = 32;   // u32[4]
 constexpr size_t kOffCursors = 48;    // u32[1]
 constexpr size_t kOffNn = 52;         // u32[1]
-constexpr size_t kOffMbr = 64;        // u32[8]
+constexpr size_t kOffMbr = 64;        // u32[32] (one MBR slot)
 constexpr size_t kOffEscIdx = 256;    // u32[64]
 constexpr size_t kOffEscMin = 512;    // f32[64]
 constexpr size_t kOffSoA = 1024;      // f32[kMaxB] x, y, z
