@@ -328,7 +328,7 @@ int ContextPinned(hd_context* ctx, size_t bytes, void** out) {
       HD_HIP_TRY(hipHostFree(ctx->host_pinned));
       ctx->host_pinned = nullptr;
     }
-    size_t want = std::max<size_t>(bytes, 4096);
+    size_t want = std::max<size_t>(bytes, kPinnedMinBytes);
     HD_HIP_TRY(hipHostMalloc(&ctx->host_pinned, want, hipHostMallocDefault));
     ctx->host_pinned_bytes = want;
   }

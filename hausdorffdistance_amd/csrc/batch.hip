@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void BatchEbKernel(BatchArgs args) {
       cmax2 = __uint_as_float(Uniform(cm_next));
     }
   }
-  if (lane == 0) atomicAdd(args.evals, evals);
+  if (lane == 0) CounterAdd(args.evals, evals);
 }
 
 struct BatchResult {
@@ -279,10 +279,11 @@ __global__ void ZeroKernel(unsigned long long* keys, uint32_t n_keys,
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_keys;
        i += gridDim.x * blockDim.x)
     keys[i] = 0ull;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *claim = 0u;
-    *evals = 0ull;
-  }
+  // The evaluation counter is a counter block (kCounterRows rows).
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kCounterWords;
+       i += gridDim.x * blockDim.x)
+    evals[i] = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *claim = 0u;
 }
 
 }  // namespace
@@ -366,18 +367,20 @@ int BuildPlan(hd_context* ctx, const float* d_a, const std::vector<uint64_t>& oa
   HD_TRY(ContextArray(ctx, "batch_by", soa, &by));
   if (ND == 3) HD_TRY(ContextArray(ctx, "batch_bz", soa, &bz));
   HD_TRY(ContextArray(ctx, "batch_keys", n_pairs, &keys));
-  HD_TRY(ContextArray(ctx, "batch_evals", 1, &evals));
+  HD_TRY(ContextArray(ctx, "batch_evals", kCounterWords, &evals));
   HD_TRY(ContextArray(ctx, "batch_claim", 1, &claim));
   HD_TRY(ContextArray(ctx, "batch_res", n_pairs, &d_res));
   HD_HIP_TRY(hipMemcpy(d_items, items.data(), sizeof(BatchItem) * items.size(),
                        hipMemcpyHostToDevice));
   HD_HIP_TRY(hipMemcpy(d_pairs, pairs.data(), sizeof(BatchPair) * n_pairs,
                        hipMemcpyHostToDevice));
+  // Results, then the evaluation counter block (64-byte aligned).
+  const size_t res_bytes = (sizeof(BatchResult) * n_pairs + 63) & ~(size_t)63;
   HD_HIP_TRY(hipHostMalloc(&plan->h_results,
-                           sizeof(BatchResult) * n_pairs + 64,
+                           res_bytes + sizeof(unsigned long long) * kCounterWords,
                            hipHostMallocDefault));
   plan->h_evals = reinterpret_cast<unsigned long long*>(
-      reinterpret_cast<char*>(plan->h_results) + sizeof(BatchResult) * n_pairs);
+      reinterpret_cast<char*>(plan->h_results) + res_bytes);
 
   // Capture: zero keys, repack B, the persistent scan, the NN pass and the
   // copy of the results to pinned host memory. No memset nodes.
@@ -406,8 +409,9 @@ int BuildPlan(hd_context* ctx, const float* d_a, const std::vector<uint64_t>& oa
                                         d_res);
   (void)hipMemcpyAsync(plan->h_results, d_res, sizeof(BatchResult) * n_pairs,
                        hipMemcpyDeviceToHost, ctx->stream);
-  (void)hipMemcpyAsync(plan->h_evals, evals, 8, hipMemcpyDeviceToHost,
-                       ctx->stream);
+  (void)hipMemcpyAsync(plan->h_evals, evals,
+                       sizeof(unsigned long long) * kCounterWords,
+                       hipMemcpyDeviceToHost, ctx->stream);
   hipError_t e = hipStreamEndCapture(ctx->stream, &plan->graph);
   if (e != hipSuccess)
     return Fail(HD_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
@@ -525,6 +529,9 @@ extern "C" int hd_batch_directed_device(hd_context* ctx, int n_dims,
     o.hd = std::sqrt(c);
   }
   out[0].time_ms = ms;
-  out[0].evals = *plan->h_evals;
+  unsigned long long evals_sum = 0;
+  for (uint32_t r = 0; r < kCounterRows; ++r)
+    evals_sum += plan->h_evals[r * kCounterRow];
+  out[0].evals = evals_sum;
   return HD_OK;
 }

@@ -383,25 +383,40 @@ int LaunchNnIndex(hd_context* ctx, int n_dims, const float* a,
   return HD_OK;
 }
 
+int ReadCounters(hd_context* ctx, const unsigned long long* d_block,
+                 unsigned long long* sums) {
+  // Through the context's pinned buffer (a request within kPinnedMinBytes
+  // never replaces it under a caller): a pageable destination costs a staged
+  // copy.
+  static_assert(kCounterWords * sizeof(unsigned long long) <= kPinnedMinBytes,
+                "counter block fits the pinned minimum");
+  void* pinned = nullptr;
+  HD_TRY(ContextPinned(ctx, kCounterWords * sizeof(unsigned long long), &pinned));
+  const unsigned long long* h = static_cast<const unsigned long long*>(pinned);
+  HD_HIP_TRY(hipMemcpyAsync(pinned, d_block, kCounterWords * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, ctx->stream));
+  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (uint32_t c = 0; c < kCounterRow; ++c) {
+    sums[c] = 0;
+    for (uint32_t r = 0; r < kCounterRows; ++r) sums[c] += h[r * kCounterRow + c];
+  }
+  return HD_OK;
+}
+
 int FinishDirected(hd_context* ctx, int n_dims, const float* a,
                    const float* b, uint64_t n_b,
                    const unsigned long long* key_dev,
                    const unsigned long long* evals_dev, int n_counters,
                    hd_directed_result* out) {
-  if (n_counters > 16) return Fail(HD_EINVAL, "too many counters");
-  void* pinned = nullptr;
-  HD_TRY(ContextPinned(ctx, 256, &pinned));
-  unsigned long long* h = static_cast<unsigned long long*>(pinned);
-  HD_HIP_TRY(hipMemcpyAsync(&h[0], key_dev, 8, hipMemcpyDeviceToHost,
-                            ctx->stream));
-  if (evals_dev && n_counters > 0)
-    HD_HIP_TRY(hipMemcpyAsync(&h[1], evals_dev, 8 * n_counters,
-                              hipMemcpyDeviceToHost, ctx->stream));
-  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  const unsigned long long key = h[0];
+  const ptrdiff_t col = evals_dev ? evals_dev - key_dev : 1;
+  if (n_counters < 0 || col < 1 || col + n_counters > (ptrdiff_t)kCounterRow)
+    return Fail(HD_EINVAL, "counters outside the key's counter block");
+  unsigned long long h[kCounterRow];
+  HD_TRY(ReadCounters(ctx, key_dev, h));
   out->evals = 0;
   if (evals_dev)
-    for (int i = 0; i < n_counters; ++i) out->evals += h[1 + i];
+    for (int i = 0; i < n_counters; ++i) out->evals += h[col + i];
+  const unsigned long long key = h[0];
   out->cmax2_bits = (uint32_t)(key >> 32);
   float c;
   std::memcpy(&c, &out->cmax2_bits, 4);
@@ -416,7 +431,9 @@ int FinishDirected(hd_context* ctx, int n_dims, const float* a,
   HD_TRY(ContextArray(ctx, "nn_index", 1, &d_nn));
   HD_TRY(LaunchNnIndex(ctx, n_dims, a, out->argmax, b, n_b, out->cmax2_bits,
                        d_nn));
-  uint32_t* hn = reinterpret_cast<uint32_t*>(&h[12]);
+  void* pinned = nullptr;
+  HD_TRY(ContextPinned(ctx, 64, &pinned));
+  uint32_t* hn = static_cast<uint32_t*>(pinned);
   HD_HIP_TRY(hipMemcpyAsync(hn, d_nn, 4, hipMemcpyDeviceToHost, ctx->stream));
   HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
   out->nn = *hn;

@@ -431,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void EbScanKernel(ScanArgs args) {
     const uint32_t seg = k % args.n_segs;
     ScanSegment<ND, P, S>(args, seg, lane, w, stg_idx, stg_min, n_stg);
   }
-  if (lane == 0) atomicAdd(args.evals, w.evals);
+  if (lane == 0) CounterAdd(args.evals, w.evals);
 }
 
 // Seed list: the first n order positions as A indices.
@@ -994,7 +994,7 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
     }
   }
   if (n_stg > 0) flush();
-  if (lane == 0) atomicAdd(args.evals, evals);
+  if (lane == 0) CounterAdd(args.evals, evals);
 }
 
 // The step at which every wave of a whole-B launch starts: fixed by the
@@ -1261,7 +1261,7 @@ __global__ __launch_bounds__(256) void EbDenseRoundKernel(DenseArgs args) {
     if (n_stg == 64u) flush();
   }
   if (n_stg > 0u) flush();
-  if (lane == 0) atomicAdd(args.evals, evals);
+  if (lane == 0) CounterAdd(args.evals, evals);
 }
 
 template <int ND>
@@ -1827,12 +1827,13 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
     HD_TRY(LearnedOrder(ctx, in, &order));
     order_mode = kOrderArray;
   }
-  // key, then one evaluation counter per stage: seed, the rounds, top
-  // completions, tail.
+  // A counter block: the key, then one evaluation counter per stage (seed,
+  // the rounds, top completions, tail).
+  static_assert(2 + kEbCounters <= (int)kCounterRow, "counter block row");
   unsigned long long* key = nullptr;
-  HD_TRY(ContextArray(ctx, "eb_key", 2 + kEbCounters, &key));
+  HD_TRY(ContextArray(ctx, "eb_key", kCounterWords, &key));
   unsigned long long* ev = key + 1;
-  HD_TRY(FillU64(ctx, key, 0ull, 2 + kEbCounters));
+  HD_TRY(FillU64(ctx, key, 0ull, kCounterWords));
   StageEnd(ctx, st_prep);
 
   if (in.flags & HD_FLAG_SINGLE_PASS) {
@@ -1890,12 +1891,9 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
                         out));
   StageEnd(ctx, st_fin);
   if (ctx->profiling) {
-    void* pinned = nullptr;
-    HD_TRY(ContextPinned(ctx, 256, &pinned));
-    unsigned long long* h = static_cast<unsigned long long*>(pinned);
-    HD_HIP_TRY(hipMemcpyAsync(h, ev, 8 * (1 + kEbCounters), hipMemcpyDeviceToHost,
-                              ctx->stream));
-    HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    unsigned long long sums[kCounterRow];
+    HD_TRY(ReadCounters(ctx, key, sums));
+    const unsigned long long* h = sums + 1;
     StageCount(ctx, "eb_seed", h[0], s0);
     for (int r = 0; r < kMaxRounds; ++r) {
       if (rs.round_points[r] == 0) continue;
@@ -1935,7 +1933,7 @@ int EbSinglePassScratchImpl(hd_context* ctx, const float* a, uint32_t n_a,
   if (n_bp > s.max_bp) return Fail(HD_EINVAL, "scratch holds %u points", s.max_bp);
   // Everything that must start at zero is cleared by fill kernels: the
   // scratch keeps the previous call's contents.
-  HD_TRY(FillU64(ctx, s.key, 0ull, 2));
+  HD_TRY(FillU64(ctx, s.key, 0ull, kCounterWords));
   HD_TRY(FillU32(ctx, s.counters, 0u, 4));
   const Feistel perm_b = MakeFeistel(n_b, (uint64_t)(int64_t)seed, 1);
   HD_TRY(GatherSoA(ctx, ND, b, n_b, n_bp, shuffle, perm_b, s.bx, s.by, s.bz));
@@ -1958,10 +1956,17 @@ int EbSinglePassScratchImpl(hd_context* ctx, const float* a, uint32_t n_a,
   args.esc_count = s.counters + 1;
   args.evals = s.key + 1;
   HD_TRY(LaunchScan<ND>(ctx, args, 0u, n_a));
+  // The key and the evaluation counter (column 1 of the counter block,
+  // summed over its rows); a host array, nothing is allocated.
   unsigned long long h[2] = {0, 0};
-  HD_HIP_TRY(hipMemcpyAsync(h, s.key, sizeof(h), hipMemcpyDeviceToHost,
-                            ctx->stream));
-  HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  {
+    unsigned long long block[kCounterWords];
+    HD_HIP_TRY(hipMemcpyAsync(block, s.key, sizeof(block), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    h[0] = block[0];
+    for (uint32_t r = 0; r < kCounterRows; ++r) h[1] += block[r * kCounterRow + 1];
+  }
   std::memset(out, 0, sizeof(*out));
   out->variant_used = HD_VARIANT_EB;
   out->evals = h[1];

@@ -359,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void GridQueryKernel(GridQueryArgs args) {
       const bool done = active && !st.stop && st.best != kFltMax;
       const unsigned long long k = done ? PackKey(st.best, idx) : 0ull;
       const unsigned long long kmax = WaveMaxU64(k);
-      if (lane == 0 && kmax != 0ull) atomicMax(args.key, kmax);
+      if (lane == 0 && kmax != 0ull) KeyMax(args.key, kmax);
     }
   }
   // Per-wave evaluation and lookup counts.
@@ -369,8 +369,8 @@ __global__ __launch_bounds__(kBlock) void GridQueryKernel(GridQueryArgs args) {
     e += __shfl_xor(e, off, 64);
     l += __shfl_xor(l, off, 64);
   }
-  if (lane == 0 && args.evals != nullptr) atomicAdd(args.evals, e);
-  if (lane == 0 && args.lookups != nullptr) atomicAdd(args.lookups, l);
+  if (lane == 0 && args.evals != nullptr) CounterAdd(args.evals, e);
+  if (lane == 0 && args.lookups != nullptr) CounterAdd(args.lookups, l);
 }
 
 }  // namespace
@@ -530,8 +530,9 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
   constexpr int kKeys = 3 + kEbCounters;
   constexpr int kLookups = 2;
   unsigned long long* key = nullptr;
-  HD_TRY(ContextArray(ctx, "grid_key", kKeys + kLookups, &key));
-  HD_TRY(FillU64(ctx, key, 0ull, kKeys + kLookups));
+  static_assert(kKeys + kLookups <= (int)kCounterRow, "counter block row");
+  HD_TRY(ContextArray(ctx, "grid_key", kCounterWords, &key));
+  HD_TRY(FillU64(ctx, key, 0ull, kCounterWords));
   unsigned int* esc_count = nullptr;
   HD_TRY(ContextArray(ctx, "grid_esc_count", 1, &esc_count));
   HD_TRY(FillU32(ctx, esc_count, 0u, 1));
@@ -644,12 +645,9 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
                         out));
   StageEnd(ctx, st_fin);
   if (ctx->profiling) {
-    void* pinned = nullptr;
-    HD_TRY(ContextPinned(ctx, 256, &pinned));
-    unsigned long long* h = static_cast<unsigned long long*>(pinned);
-    HD_HIP_TRY(hipMemcpyAsync(h, key + 1, 8 * (kKeys - 1 + kLookups),
-                              hipMemcpyDeviceToHost, ctx->stream));
-    HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    unsigned long long sums[kCounterRow];
+    HD_TRY(ReadCounters(ctx, key, sums));
+    const unsigned long long* h = sums + 1;
     StageCount(ctx, "grid_seed", h[0], s0, h[kKeys - 1]);
     StageCount(ctx, "grid_query", h[1], n_a - s0, h[kKeys]);
     StageCount(ctx, "grid_build", 0, in.n_b);

@@ -65,6 +65,32 @@ HD_DEV float LoadCmax2(const unsigned long long* key) {
 
 HD_DEV uint32_t LaneId() { return threadIdx.x & 63u; }
 
+// Raises the packed key to at least v. The key only grows, so whatever a
+// load returns is a lower bound of its final value: a candidate that does
+// not exceed it cannot change the result, and its atomic (which would queue
+// behind every other wave's on the one word) is skipped.
+HD_DEV void KeyMax(unsigned long long* key, unsigned long long v) {
+  if (v > __hip_atomic_load(key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(key, v);
+}
+
+// Event counters (evaluations, lookups) live in blocks of kCounterRows rows
+// of kCounterRow 64-bit words: a kernel adds its totals into row
+// (blockIdx.x mod kCounterRows) of its counter's column, so the final
+// atomics of thousands of waves spread over as many cache lines instead of
+// queueing on one address (a device-scope atomic on one address is served
+// one at a time: 16k of them cost the 10M grid query 0.6 ms). The host sums
+// the rows (ReadCounters, FinishDirected). A counter block is allocated and
+// cleared whole; its counters are columns of row 0, and column 0 of row 0
+// may hold the packed result key, which only atomicMax touches.
+constexpr uint32_t kCounterRow = 16;
+constexpr uint32_t kCounterRows = 32;
+constexpr uint32_t kCounterWords = kCounterRow * kCounterRows;
+
+HD_DEV void CounterAdd(unsigned long long* c, unsigned long long v) {
+  atomicAdd(c + kCounterRow * (blockIdx.x % kCounterRows), v);
+}
+
 HD_DEV uint32_t Uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 HD_DEV float Uniform(float v) {
   return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));

@@ -137,7 +137,11 @@ int ContextArray(hd_context* ctx, const char* name, size_t count, T** out) {
   return rc;
 }
 
-// Pinned host staging of at least `bytes`.
+// Pinned host staging of at least `bytes`. The buffer is never smaller than
+// kPinnedMinBytes, so a request up to that size returns the same buffer as
+// before (pointers that callers still hold stay valid); a larger request may
+// replace it.
+constexpr size_t kPinnedMinBytes = 4096;
 int ContextPinned(hd_context* ctx, size_t bytes, void** out);
 
 // Makes ctx->device current on the calling thread.
@@ -288,9 +292,15 @@ int NearestNeighbors(hd_context* ctx, int index, int n_dims, const float* q,
 int LearnedOrder(hd_context* ctx, const DirectedArgs& args,
                  const uint32_t** order_out);
 
-// Finishes a directed result from the device key: reads it and the
-// n_counters evaluation counters at evals_dev (summed), and finds the NN
-// index of the arg-max.
+// Sums of the kCounterRow columns of the counter block at d_block (one
+// synchronous copy of kCounterWords words).
+int ReadCounters(hd_context* ctx, const unsigned long long* d_block,
+                 unsigned long long* sums);
+
+// Finishes a directed result from the device counter block key_dev: reads
+// the key (word 0) and the n_counters evaluation counters at evals_dev (in
+// the same block, summed over its rows), and finds the NN index of the
+// arg-max.
 int FinishDirected(hd_context* ctx, int n_dims, const float* a,
                    const float* b, uint64_t n_b,
                    const unsigned long long* key_dev,

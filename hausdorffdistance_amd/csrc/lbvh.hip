@@ -445,7 +445,7 @@ __global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
       const bool done = active && !broke && !overflow && best != kFltMax;
       const unsigned long long k = done ? PackKey(best, idx) : 0ull;
       const unsigned long long kmax = WaveMaxU64(k);
-      if (lane == 0 && kmax != 0ull) atomicMax(args.key, kmax);
+      if (lane == 0 && kmax != 0ull) KeyMax(args.key, kmax);
     }
   }
   unsigned long long e = evals, l = nodes;
@@ -454,8 +454,8 @@ __global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
     e += __shfl_xor(e, off, 64);
     l += __shfl_xor(l, off, 64);
   }
-  if (lane == 0 && args.evals != nullptr) atomicAdd(args.evals, e);
-  if (lane == 0 && args.lookups != nullptr) atomicAdd(args.lookups, l);
+  if (lane == 0 && args.evals != nullptr) CounterAdd(args.evals, e);
+  if (lane == 0 && args.lookups != nullptr) CounterAdd(args.lookups, l);
 }
 
 }  // namespace
@@ -662,8 +662,8 @@ int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
 
   unsigned long long* key = nullptr;
   // key, evaluations of seed and query, then their node-record reads.
-  HD_TRY(ContextArray(ctx, "lbvh_key", 6, &key));
-  HD_TRY(FillU64(ctx, key, 0ull, 6));
+  HD_TRY(ContextArray(ctx, "lbvh_key", kCounterWords, &key));
+  HD_TRY(FillU64(ctx, key, 0ull, kCounterWords));
   unsigned int* esc_count = nullptr;
   uint32_t* esc_idx = nullptr;
   float* esc_min = nullptr;
@@ -758,10 +758,9 @@ int RunLbvhImpl(hd_context* ctx, const DirectedArgs& in,
   HD_TRY(FinishDirected(ctx, ND, in.a, in.b, in.n_b, key, key + 1, 2, out));
   StageEnd(ctx, st_fin);
   if (ctx->profiling) {
-    unsigned long long* hh = static_cast<unsigned long long*>(pinned);
-    HD_HIP_TRY(hipMemcpyAsync(hh, key + 1, 40, hipMemcpyDeviceToHost,
-                              ctx->stream));
-    HD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    unsigned long long sums[kCounterRow];
+    HD_TRY(ReadCounters(ctx, key, sums));
+    const unsigned long long* hh = sums + 1;
     StageCount(ctx, "lbvh_seed", hh[0], s0, hh[3]);
     StageCount(ctx, "lbvh_query", hh[1], n_a - s0, hh[4]);
     StageCount(ctx, "lbvh_overflow", 0, n_over);

@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void MaxOfMinKernel(
   __syncthreads();
   if (threadIdx.x == 0) {
     best = max(max(part[0], part[1]), max(part[2], part[3]));
-    if (best != 0ull) atomicMax(key, best);
+    if (best != 0ull) dev::KeyMax(key, best);
   }
 }
 
@@ -125,8 +125,8 @@ int RunNearestNeighborSearch(hd_context* ctx, const DirectedArgs& args,
   unsigned long long* key = nullptr;
   HD_TRY(ContextArray(ctx, "nns_d2", args.n_a, &d2));
   HD_TRY(ContextArray(ctx, "nns_idx", args.n_a, &idx));
-  HD_TRY(ContextArray(ctx, "nns_key", 2, &key));
-  HD_TRY(FillU64(ctx, key, 0ull, 1));
+  HD_TRY(ContextArray(ctx, "nns_key", kCounterWords, &key));
+  HD_TRY(FillU64(ctx, key, 0ull, kCounterWords));
   HD_TRY(FillU64(ctx, key + 1, (unsigned long long)args.n_a, 1));
   HD_TRY(GridNearestNeighbors(ctx, args.n_dims, args.a, args.n_a, args.b,
                               args.n_b, d2, idx));
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void ExactMinListKernel(
         out_idx[idx] = bj;
       }
       if (key != nullptr && best != dev::kFltMax)
-        atomicMax(key, dev::PackKey(best, idx));
+        dev::KeyMax(key, dev::PackKey(best, idx));
     }
   }
 }

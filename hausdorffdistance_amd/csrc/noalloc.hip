@@ -19,6 +19,7 @@
 #include <mutex>
 
 #include "hd.h"
+#include "hd_device.h"
 #include "hd_internal.h"
 #include "hd_primitives.h"
 
@@ -29,17 +30,20 @@ namespace {
 constexpr uint64_t kMaxB = 1ull << 20;  // B points (2^20 is a whole number of
                                         // 64-point steps: no padding beyond)
 // Byte offsets inside the array.
-constexpr size_t kOffKey = 0;         // u64 key, u64 evaluation counter
-constexpr size_t kOffCounters = 32;   // u32[4]
-constexpr size_t kOffCursors = 48;    // u32[1]
-constexpr size_t kOffNn = 52;         // u32[1]
-constexpr size_t kOffMbr = 64;        // u32[32] (one MBR slot)
-constexpr size_t kOffEscIdx = 256;    // u32[64]
-constexpr size_t kOffEscMin = 512;    // f32[64]
-constexpr size_t kOffSoA = 1024;      // f32[kMaxB] x, y, z
+constexpr size_t kOffKey = 0;         // counter block: u64[kCounterWords]
+                                      // (key, evaluation counter), 4 KiB
+constexpr size_t kOffCounters = 4096; // u32[4]
+constexpr size_t kOffCursors = 4112;  // u32[1]
+constexpr size_t kOffNn = 4116;       // u32[1]
+constexpr size_t kOffMbr = 4224;      // u32[32] (one MBR slot)
+constexpr size_t kOffEscIdx = 4352;   // u32[64]
+constexpr size_t kOffEscMin = 4608;   // f32[64]
+constexpr size_t kOffSoA = 5120;      // f32[kMaxB] x, y, z
+static_assert(kOffKey + sizeof(unsigned long long) * dev::kCounterWords <= kOffCounters,
+              "counter block");
 constexpr size_t kScratchBytes = kOffSoA + 3 * kMaxB * sizeof(float);
 
-// 12 MiB + 1 KiB per GPU. In host code this name stands for the array but is
+// 12 MiB + 5 KiB per GPU. In host code this name stands for the array but is
 // not its device address: kernels and copies use the address that
 // hipGetSymbolAddress returns for the current device.
 __device__ __align__(256) unsigned char scratch[kScratchBytes];

@@ -248,8 +248,8 @@ __global__ __launch_bounds__(256) void RtHdistScanKernel(
   }
   // One atomic per wave (an atomic per A point on the one key serialises:
   // 1M of them cost ~10 ms). kmax and my_evals are the same in every lane.
-  if (lane == 0 && kmax != 0ull) atomicMax(key, kmax);
-  if (lane == 0 && my_evals) atomicAdd(evals, my_evals);
+  if (lane == 0 && kmax != 0ull) KeyMax(key, kmax);
+  if (lane == 0 && my_evals) CounterAdd(evals, my_evals);
 }
 
 template <int ND>
@@ -417,8 +417,8 @@ int RunRtHdist(hd_context* ctx, const DirectedArgs& args, hd_directed_result* ou
 
   const int stage_scan = StageBegin(ctx, "rthdist_scan");
   unsigned long long* key = nullptr;
-  HD_TRY(ContextArray(ctx, "rth_key", 2, &key));
-  HD_TRY(FillU64(ctx, key, 0ull, 2));
+  HD_TRY(ContextArray(ctx, "rth_key", kCounterWords, &key));
+  HD_TRY(FillU64(ctx, key, 0ull, kCounterWords));
   const int scan_blocks = (int)std::min<uint64_t>((n_a + 3) / 4, (uint64_t)ctx->cu_count * 8);
   if (nd == 2)
     RtHdistScanKernel<2><<<scan_blocks, 256, 0, ctx->stream>>>(

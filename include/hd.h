@@ -232,7 +232,7 @@ int hd_directed_host(int n_dims, const float* a, uint64_t n_a, const float* b,
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc). The early break of hd_directed_device with
  * HD_FLAG_SINGLE_PASS and the same kernels, but every buffer (key, counters,
- * B repacked) lives in one device array of the library, 12 MiB + 1 KiB per
+ * B repacked) lives in one device array of the library, 12 MiB + 5 KiB per
  * GPU, held until the process ends; the call makes no allocation of any
  * kind. Limits: |B| <= 1,048,576 points (HD_EINVAL above); ctx must be a
  * context made beforehand (NULL is HD_EINVAL); flags take NO_SHUFFLE,
