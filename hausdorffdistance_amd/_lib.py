@@ -36,6 +36,8 @@ HD_FLAG_COUNT_EVALS = 0x8
 HD_FLAG_INPUT_ON_DEVICE = 0x10
 HD_FLAG_NO_SHUFFLE = 0x20
 HD_FLAG_SINGLE_PASS = 0x40
+HD_FLAG_B_ORDER_RANDOM = 0x80
+HD_FLAG_B_ORDER_STRATIFIED = 0x100
 HD_AUTO_TUNE_ALL = 0x1
 HD_AUTO_TUNE_N_POINTS_CELL = 0x2
 HD_AUTO_TUNE_MAX_HIT = 0x4

@@ -58,6 +58,14 @@ class Execution(IntEnum):
         raise HDError(_lib.HD_EINVAL, f"Unknown execution: {s}")
 
 
+def _b_order_flags(b_order: str) -> int:
+    table = {"auto": 0, "random": _lib.HD_FLAG_B_ORDER_RANDOM,
+             "stratified": _lib.HD_FLAG_B_ORDER_STRATIFIED}
+    if b_order not in table:
+        raise HDError(_lib.HD_EINVAL, f"Unknown b_order: {b_order}")
+    return table[b_order]
+
+
 @dataclass
 class RunConfig:
     """Reference RunConfig restricted to the hot path; defaults of
@@ -76,6 +84,7 @@ class RunConfig:
     symmetric: bool = False
     learned_order: bool = False
     shuffle: bool = True
+    b_order: str = "auto"              # EB visit order of B: auto, random, stratified
     sample_rate: float = 0.01
     n_points_cell: int = 8
     max_hit: int = 256
@@ -103,6 +112,7 @@ class RunConfig:
             flags |= _lib.HD_FLAG_LEARNED_ORDER
         if not self.shuffle:
             flags |= _lib.HD_FLAG_NO_SHUFFLE
+        flags |= _b_order_flags(self.b_order)
         c.flags = flags
         c.sample_rate = float(self.sample_rate)
         c.n_points_cell = int(self.n_points_cell)
@@ -312,10 +322,15 @@ def run_hausdorff_distance(config: RunConfig, points_a, points_b,
 def directed_device(ctx: Context, variant: Variant, points_a, points_b, *,
                     n_dims: Optional[int] = None, seed: int = 0, shuffle: bool = True,
                     learned_order: bool = False, model=None,
-                    single_pass: bool = False) -> DirectedResult:
+                    single_pass: bool = False,
+                    b_order: str = "auto") -> DirectedResult:
     """h(A -> B) on device-resident (torch CUDA) inputs. single_pass (early
     break only): one pass over A in the visit order with no seed, rounds or
-    filter, the reference GPU kernel's schedule (a comparison baseline)."""
+    filter, the reference GPU kernel's schedule (a comparison baseline).
+    b_order (early break): the visit order of B, "random" (pseudo-random,
+    as the reference shuffles it), "stratified" (Morton ranks spread evenly
+    over the sequence) or "auto" (stratified with learned_order, else
+    random); the result is the same in every order."""
     nd = n_dims or int(points_a.shape[1])
     pa, na, da, _ = _points(points_a, nd)
     pb, nb, db, _ = _points(points_b, nd)
@@ -328,6 +343,7 @@ def directed_device(ctx: Context, variant: Variant, points_a, points_b, *,
         flags |= _lib.HD_FLAG_LEARNED_ORDER
     if single_pass:
         flags |= _lib.HD_FLAG_SINGLE_PASS
+    flags |= _b_order_flags(b_order)
     out = _lib.DirectedResultC()
     mh = getattr(model, "handle", None)
     _lib.check(_lib.load().hd_directed_device(ctx.handle, int(variant), nd, pa, na, pb, nb,

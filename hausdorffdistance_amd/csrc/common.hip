@@ -38,12 +38,16 @@ __global__ void IotaKernel(uint32_t* p, uint32_t n) {
 template <int ND>
 __global__ void GatherSoAKernel(const float* __restrict__ aos, uint32_t n,
                                 uint32_t n_pad, int shuffle, Feistel perm,
+                                const uint32_t* __restrict__ idx,
                                 float* __restrict__ x, float* __restrict__ y,
                                 float* __restrict__ z) {
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_pad;
        j += gridDim.x * blockDim.x) {
     uint32_t src = j < n ? j : j % n;
-    if (shuffle) src = Permute(perm, src);
+    if (idx != nullptr)
+      src = idx[src];
+    else if (shuffle)
+      src = Permute(perm, src);
     const float* p = aos + (uint64_t)src * ND;
     x[j] = p[0];
     y[j] = p[1];
@@ -268,15 +272,15 @@ int FillU64(hd_context* ctx, unsigned long long* p, unsigned long long value,
 
 int GatherSoA(hd_context* ctx, int n_dims, const float* aos, uint32_t n,
               uint32_t n_pad, bool shuffle, const Feistel& perm, float* x,
-              float* y, float* z) {
+              float* y, float* z, const uint32_t* idx) {
   if (n == 0 || n_pad < n) return Fail(HD_EINVAL, "GatherSoA: bad sizes");
   const int grid = GridFor(n_pad, 256, 8192);
   if (n_dims == 2)
     GatherSoAKernel<2><<<grid, 256, 0, ctx->stream>>>(aos, n, n_pad, shuffle,
-                                                      perm, x, y, z);
+                                                      perm, idx, x, y, z);
   else
     GatherSoAKernel<3><<<grid, 256, 0, ctx->stream>>>(aos, n, n_pad, shuffle,
-                                                      perm, x, y, z);
+                                                      perm, idx, x, y, z);
   HD_HIP_TRY(hipGetLastError());
   return HD_OK;
 }

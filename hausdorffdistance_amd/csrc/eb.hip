@@ -39,6 +39,7 @@
 
 #include "hd_device.h"
 #include "hd_internal.h"
+#include "hd_lbvh.h"
 #include "hd_primitives.h"
 
 namespace hd {
@@ -1634,12 +1635,36 @@ bool UseFilter(const EbTarget& t, uint32_t budget, float cmax2) {
          t.err < 0.5f * cmax2;
 }
 
-// B repacked for the scans: SoA, pseudo-randomly permuted (unless the
-// caller asks for input order), padded to whole 64-point steps with copies
-// of existing points (duplicates never change a minimum).
+// Visit orders of B in the scans.
+enum class BOrder { kInput, kPseudoRandom, kStratified };
+
+// The order the flags select. By default the learned pipeline takes the
+// stratified order and the other schedules the pseudo-random one: the
+// ranked plain pipeline ranks A by partial minima over the first steps of
+// B, and an evenly spread sample ranks the points of sparse regions first,
+// which rarely define the HD (10M GMM: 1.7x the evaluations of the
+// pseudo-random order), while the learned order's seed reaches the final
+// cmax2 before any round (10M GMM: 12.5% fewer evaluations).
+BOrder BOrderFor(uint32_t flags, uint32_t n_b) {
+  if (flags & HD_FLAG_NO_SHUFFLE) return BOrder::kInput;
+  if (n_b > kMaxStratified || (flags & HD_FLAG_B_ORDER_RANDOM))
+    return BOrder::kPseudoRandom;
+  if (flags & HD_FLAG_B_ORDER_STRATIFIED) return BOrder::kStratified;
+  return (flags & HD_FLAG_LEARNED_ORDER) && !(flags & HD_FLAG_SINGLE_PASS)
+             ? BOrder::kStratified
+             : BOrder::kPseudoRandom;
+}
+
+// B repacked for the scans: SoA in the visit order, padded to whole 64-point
+// steps with copies of existing points (duplicates never change a minimum).
+// The stratified order (StratifiedOrder) spreads the B points of every
+// small region evenly over the sequence, so a scan from any step meets the
+// points near its A point sooner than in a pseudo-random order, where they
+// fall at random (1M GMM, CPU model with the final cmax2 and one scan per A
+// point from a random start: 3.2k evaluations per A point against 4.0k).
 template <int ND>
-int PrepareB(hd_context* ctx, const float* b, uint32_t n_b, bool shuffle,
-             int32_t seed, EbTarget* t) {
+int PrepareB(hd_context* ctx, const float* b, uint32_t n_b, BOrder order,
+             int32_t seed, const float* mbr_b, EbTarget* t) {
   t->n_bp = (n_b + 63u) & ~63u;
   t->n_steps = t->n_bp / 64u;
   t->bx = t->by = t->bz = nullptr;
@@ -1647,8 +1672,11 @@ int PrepareB(hd_context* ctx, const float* b, uint32_t n_b, bool shuffle,
   HD_TRY(ContextArray(ctx, "eb_by", t->n_bp, &t->by));
   if (ND == 3) HD_TRY(ContextArray(ctx, "eb_bz", t->n_bp, &t->bz));
   const Feistel perm_b = MakeFeistel(n_b, (uint64_t)(int64_t)seed, 1);
-  return GatherSoA(ctx, ND, b, n_b, t->n_bp, shuffle, perm_b, t->bx, t->by,
-                   t->bz);
+  uint32_t* idx = nullptr;
+  if (order == BOrder::kStratified)
+    HD_TRY(StratifiedOrder(ctx, ND, b, n_b, mbr_b, &idx));
+  return GatherSoA(ctx, ND, b, n_b, t->n_bp, order == BOrder::kPseudoRandom,
+                   perm_b, t->bx, t->by, t->bz, idx);
 }
 
 // Rounds over the points still open: slots with a growing step budget;
@@ -1816,7 +1844,8 @@ int RunEarlyBreakImpl(hd_context* ctx, const DirectedArgs& in,
 
   const int st_prep = StageBegin(ctx, "eb_prep");
   EbTarget t;
-  HD_TRY(PrepareB<ND>(ctx, in.b, n_b, shuffle, in.seed, &t));
+  HD_TRY(PrepareB<ND>(ctx, in.b, n_b, BOrderFor(in.flags, n_b), in.seed,
+                      in.has_mbr ? in.mbr_b : nullptr, &t));
   if (in.has_mbr && t.n_steps > 16u)
     HD_TRY(PrepareFilter<ND>(ctx, in.mbr_a, in.mbr_b, &t));
   // Visit order of A: learned (array), pseudo-random (Feistel) or input.
@@ -2003,10 +2032,10 @@ int FilterProbe(hd_context* ctx, int n_dims, const float* a, uint32_t n_a,
   HD_TRY(HostMbrPair(ctx, n_dims, a, n_a, mbr_a, b, n_b, mbr_b));
   EbTarget t;
   if (n_dims == 2) {
-    HD_TRY(PrepareB<2>(ctx, b, n_b, false, 0, &t));
+    HD_TRY(PrepareB<2>(ctx, b, n_b, BOrder::kInput, 0, nullptr, &t));
     HD_TRY(PrepareFilter<2>(ctx, mbr_a, mbr_b, &t));
   } else {
-    HD_TRY(PrepareB<3>(ctx, b, n_b, false, 0, &t));
+    HD_TRY(PrepareB<3>(ctx, b, n_b, BOrder::kInput, 0, nullptr, &t));
     HD_TRY(PrepareFilter<3>(ctx, mbr_a, mbr_b, &t));
   }
   if (t.bt == nullptr)
@@ -2039,9 +2068,11 @@ int RunEarlyBreak(hd_context* ctx, const DirectedArgs& args,
 }
 
 int EbPrepareTarget(hd_context* ctx, int n_dims, const float* b, uint32_t n_b,
-                    bool shuffle, int32_t seed, EbTarget* t) {
-  if (n_dims == 2) return PrepareB<2>(ctx, b, n_b, shuffle, seed, t);
-  return PrepareB<3>(ctx, b, n_b, shuffle, seed, t);
+                    uint32_t flags, int32_t seed, const float* mbr_b,
+                    EbTarget* t) {
+  const BOrder order = BOrderFor(flags, n_b);
+  if (n_dims == 2) return PrepareB<2>(ctx, b, n_b, order, seed, mbr_b, t);
+  return PrepareB<3>(ctx, b, n_b, order, seed, mbr_b, t);
 }
 
 int EbFinishList(hd_context* ctx, int n_dims, const float* a, uint32_t n_a,

@@ -632,8 +632,13 @@ int RunGridImpl(hd_context* ctx, const DirectedArgs& in, uint32_t max_cells,
     if (n_esc > 0) {
       const int st_eb = StageBegin(ctx, "hybrid_eb");
       EbTarget t;
-      HD_TRY(EbPrepareTarget(ctx, ND, in.b, (uint32_t)in.n_b,
-                             (in.flags & HD_FLAG_NO_SHUFFLE) == 0, in.seed, &t));
+      // B pseudo-randomly ordered unless the caller asks otherwise: the
+      // escalated points are the far ones, which an evenly spread order
+      // does not reach sooner, and the stratified order costs a sort of B.
+      const uint32_t order_flags =
+          in.flags & (HD_FLAG_NO_SHUFFLE | HD_FLAG_B_ORDER_STRATIFIED);
+      HD_TRY(EbPrepareTarget(ctx, ND, in.b, (uint32_t)in.n_b, order_flags,
+                             in.seed, in.has_mbr ? in.mbr_b : nullptr, &t));
       EbRoundStats rs;
       HD_TRY(EbFinishList(ctx, ND, in.a, n_a, t, esc_idx, esc_min, n_esc, key,
                           key + 3, &rs));

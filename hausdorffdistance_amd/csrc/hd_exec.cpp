@@ -275,7 +275,7 @@ int main(int argc, char** argv) {
             "-n_dims 2|3 [-variant eb|grid|rt|lbvh|hybrid|nn|rt-hdist] [-execution gpu|cpu] "
             "[-repeat 5] [-seed 0] [-parallelism -1] [-check|-nocheck] [-symmetric] [-learned_order] "
             "[-n_points_cell 8] [-max_hit 256] [-auto_tune] [-auto_tune_n_points_cell] "
-            "[-auto_tune_max_hit] [-limit N] "
+            "[-auto_tune_max_hit] [-b_order auto|random|stratified] [-limit N] "
             "[-stats_n_points_cell 10] [-vary_params -sample_rate_list r,.. "
             "-eb_only_threshold_list n,.. -max_hit_list n,.. -n_points_cell_list n,.. "
             "[-overwrite]] "
@@ -326,6 +326,17 @@ int main(int argc, char** argv) {
   if (f.Bool("symmetric", false)) cfg.flags |= HD_FLAG_SYMMETRIC;
   if (f.Bool("learned_order", false)) cfg.flags |= HD_FLAG_LEARNED_ORDER;
   if (!f.Bool("shuffle", true)) cfg.flags |= HD_FLAG_NO_SHUFFLE;
+  // Early break: the visit order of B (auto: stratified with the learned
+  // order, else pseudo-random as the reference shuffles it; same result).
+  const std::string b_order = f.Get("b_order", "auto");
+  if (b_order == "random")
+    cfg.flags |= HD_FLAG_B_ORDER_RANDOM;
+  else if (b_order == "stratified")
+    cfg.flags |= HD_FLAG_B_ORDER_STRATIFIED;
+  else if (b_order != "auto") {
+    fprintf(stderr, "Unknown b_order: %s (auto, random or stratified)\n", b_order.c_str());
+    return 1;
+  }
 
   const long limit = f.Int("limit", 0x7FFFFFFFL);
   const std::string type = f.Get("input_type", "");

@@ -258,8 +258,13 @@ struct EbRoundStats {
   uint64_t rest_points = 0;     // order positions the sampled first round left
   bool rest_filtered = false;
 };
+// B repacked for the early-break scans in the visit order that `flags`
+// select (HD_FLAG_NO_SHUFFLE: input order; HD_FLAG_RANDOM_B_ORDER or
+// HD_FLAG_SINGLE_PASS: pseudo-random; else spatially stratified). mbr_b:
+// B's MBR when the caller has it, else null.
 int EbPrepareTarget(hd_context* ctx, int n_dims, const float* b, uint32_t n_b,
-                    bool shuffle, int32_t seed, EbTarget* t);
+                    uint32_t flags, int32_t seed, const float* mbr_b,
+                    EbTarget* t);
 // Finishes the listed A points (indices, with optional partial minima over
 // a subset of B) exactly with early break, raising *key; ev[0..kEbCounters)
 // count evaluations of the rounds, the top completions and the tail.

@@ -71,11 +71,26 @@ struct LbvhQueryArgs {
 int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
               uint32_t leaf_size, LbvhIndex* out, const float* mbr_b = nullptr);
 int LaunchLbvhQuery(hd_context* ctx, const LbvhQueryArgs& args);
+// Quantisation frame over an MBR (2 n_dims floats) for the Morton codes.
+LbvhFrame MortonFrame(int n_dims, const float* mbr);
 // Permutation of pts sorted by bits [begin_bit, 32) of their 32-bit Morton
 // codes in frame f, equal keys in input order (device array owned by the
-// context).
+// context; `tag` prefixes the names of its buffers, so callers whose orders
+// must coexist pass different tags).
 int MortonOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
-                const LbvhFrame& f, uint32_t** order_out, int begin_bit = 0);
+                const LbvhFrame& f, uint32_t** order_out, int begin_bit = 0,
+                const char* tag = "lbvh_q", uint32_t out_mul = 0);
+// A spatially stratified visit order of pts (perm[j] = index of the point
+// visited j-th; device array owned by the context): the points ranked by
+// the top 16 bits of their Morton code in their MBR (equal codes in input
+// order), rank r at position r * step mod n with step coprime with n and
+// near n / phi, so every run of consecutive ranks (the points of one
+// region) is spread evenly over the sequence. mbr: pts' MBR when the caller
+// has it, else null. (MortonOrder with out_mul: the order's entry of rank k
+// is written to position k * out_mul mod n.)
+constexpr uint32_t kMaxStratified = 1u << 31;
+int StratifiedOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
+                    const float* mbr, uint32_t** perm_out);
 // start[p] = leaf of tree holding the first B code >= the Morton code of
 // query q[order[p]] (clamped to the last leaf); order should be a Morton
 // order of the queries, for coherent searches.

@@ -12,14 +12,16 @@ namespace hd {
 // Stable radix sorts of (key, u32 value) pairs (sort.hip): ascending on key
 // bits [begin_bit, end_bit) for u32 (the other bits are carried), on all 16
 // bits for u16, descending for finite f32 keys. Inputs and outputs must not
-// alias.
+// alias. u32 with out_mul (coprime with n): the value of rank k goes to
+// v_out[k * out_mul mod n] and k_out receives no sorted keys (it may still
+// serve as scratch).
 int SortPairsDescF32(hd_context* ctx, const float* k_in, float* k_out,
                      const uint32_t* v_in, uint32_t* v_out, uint32_t n);
 int SortPairsU16(hd_context* ctx, const uint16_t* k_in, uint16_t* k_out,
                  const uint32_t* v_in, uint32_t* v_out, uint32_t n);
 int SortPairsU32(hd_context* ctx, const uint32_t* k_in, uint32_t* k_out,
                  const uint32_t* v_in, uint32_t* v_out, uint32_t n,
-                 int end_bit, int begin_bit = 0);
+                 int end_bit, int begin_bit = 0, uint32_t out_mul = 0);
 
 // Host construction of a Feistel permutation of [0, n) (hd_device.h).
 Feistel MakeFeistel(uint32_t n, uint64_t seed, uint32_t stream_id);
@@ -38,12 +40,13 @@ int FillU32(hd_context* ctx, uint32_t* p, uint32_t value, uint64_t n);
 int FillU64(hd_context* ctx, unsigned long long* p, unsigned long long value,
             uint64_t n);
 
-// Repacks AoS points into SoA arrays of n_pad entries (n_pad >= n), visiting
-// source index perm(j mod n) for destination j (identity if !shuffle).
+// Repacks AoS points into SoA arrays of n_pad entries (n_pad >= n),
+// visiting source index idx[j mod n] for destination j when idx (a device
+// array of n indices) is given, else perm(j mod n) (identity if !shuffle).
 // z may be null for 2D.
 int GatherSoA(hd_context* ctx, int n_dims, const float* aos, uint32_t n,
               uint32_t n_pad, bool shuffle, const Feistel& perm,
-              float* x, float* y, float* z);
+              float* x, float* y, float* z, const uint32_t* idx = nullptr);
 
 // Reads the MBR back (synchronous) and applies the reference's
 // ToNonemptyMBR widening (src/geoms/mbr.h:288-296). Non-finite coordinates

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "hd_device.h"
 #include "hd_internal.h"
@@ -460,22 +461,73 @@ __global__ __launch_bounds__(64) void LbvhQueryKernel(LbvhQueryArgs args) {
 
 }  // namespace
 
+LbvhFrame MortonFrame(int n_dims, const float* mbr) {
+  LbvhFrame f;
+  std::memset(&f, 0, sizeof(f));
+  // Largest quantisation level, exactly representable in FP32 so the
+  // clamp in MortonKernel never converts an out-of-range float.
+  const double levels = n_dims == 2 ? 4294967040.0 : 2097151.0;
+  f.qmax = (float)levels;
+  for (int d = 0; d < n_dims; ++d) {
+    f.lo[d] = mbr[d];
+    // A zero extent (a flat set whose widening rounded away) quantises to 0.
+    const double ext = (double)mbr[n_dims + d] - (double)mbr[d];
+    f.scale[d] = ext > 0 ? (float)(levels / ext) : 0.f;
+  }
+  return f;
+}
+
 int MortonOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
-                const LbvhFrame& f, uint32_t** order_out, int begin_bit) {
+                const LbvhFrame& f, uint32_t** order_out, int begin_bit,
+                const char* tag, uint32_t out_mul) {
   uint32_t *keys = nullptr, *keys_s = nullptr;
   uint32_t *vals = nullptr, *order = nullptr;
-  HD_TRY(ContextArray(ctx, "lbvh_qkeys", n, &keys));
-  HD_TRY(ContextArray(ctx, "lbvh_qkeys_s", n, &keys_s));
-  HD_TRY(ContextArray(ctx, "lbvh_qvals", n, &vals));
-  HD_TRY(ContextArray(ctx, "lbvh_qorder", n, &order));
+  const std::string t(tag);
+  HD_TRY(ContextArray(ctx, (t + "keys").c_str(), n, &keys));
+  HD_TRY(ContextArray(ctx, (t + "keys_s").c_str(), n, &keys_s));
+  HD_TRY(ContextArray(ctx, (t + "vals").c_str(), n, &vals));
+  HD_TRY(ContextArray(ctx, (t + "order").c_str(), n, &order));
   const int blocks = (int)std::min<uint64_t>((n + 255) / 256, 8192);
   if (n_dims == 2)
     MortonKernel32<2><<<blocks, 256, 0, ctx->stream>>>(pts, n, f, keys, vals);
   else
     MortonKernel32<3><<<blocks, 256, 0, ctx->stream>>>(pts, n, f, keys, vals);
   HD_HIP_TRY(hipGetLastError());
-  HD_TRY(SortPairsU32(ctx, keys, keys_s, vals, order, n, 32, begin_bit));
+  HD_TRY(SortPairsU32(ctx, keys, keys_s, vals, order, n, 32, begin_bit,
+                     out_mul));
   *order_out = order;
+  return HD_OK;
+}
+
+int StratifiedOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
+                    const float* mbr, uint32_t** perm_out) {
+  if (n == 0 || n > kMaxStratified)
+    return Fail(HD_EINVAL, "StratifiedOrder: %u points", n);
+  float m[6];
+  if (mbr)
+    std::memcpy(m, mbr, sizeof(float) * 2 * n_dims);
+  else
+    HD_TRY(HostMbr(ctx, n_dims, pts, n, m));
+  // Rank r goes to position r * step mod n, step coprime with n and near
+  // n / phi: the ranks of any run (the points of one region) land evenly
+  // spread over the sequence (the gaps of an irrational rotation take at
+  // most three values).
+  uint64_t step = std::max<uint64_t>(1, (uint64_t)std::llround((double)n * 0.6180339887498949));
+  auto gcd = [](uint64_t x, uint64_t y) {
+    while (y) {
+      const uint64_t t = x % y;
+      x = y;
+      y = t;
+    }
+    return x;
+  };
+  while (n > 1 && gcd(step, n) != 1) ++step;
+  // Ranks by the top 16 bits of the Morton code (cells of 2^-8 of the
+  // extent per axis in 2D, about 2^-5 in 3D; two radix passes: a CPU model
+  // of the 1M GMM pair gives the same evaluations as with 24 bits); the
+  // sort's last pass writes each index straight to its position.
+  HD_TRY(MortonOrder(ctx, n_dims, pts, n, MortonFrame(n_dims, m), perm_out,
+                     16, "border_", n > 1 ? (uint32_t)step : 0u));
   return HD_OK;
 }
 
@@ -505,18 +557,7 @@ int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
     std::memcpy(mbr, mbr_b, sizeof(float) * 2 * nd);
   else
     HD_TRY(HostMbr(ctx, nd, b, n_b, mbr));
-  LbvhFrame f;
-  std::memset(&f, 0, sizeof(f));
-  // Largest quantisation level, exactly representable in FP32 so the
-  // clamp in MortonKernel never converts an out-of-range float.
-  const double levels = nd == 2 ? 4294967040.0 : 2097151.0;
-  f.qmax = (float)levels;
-  for (int d = 0; d < nd; ++d) {
-    f.lo[d] = mbr[d];
-    // A zero extent (a flat set whose widening rounded away) quantises to 0.
-    const double ext = (double)mbr[nd + d] - (double)mbr[d];
-    f.scale[d] = ext > 0 ? (float)(levels / ext) : 0.f;
-  }
+  const LbvhFrame f = MortonFrame(nd, mbr);
   uint32_t *keys = nullptr, *keys_s = nullptr;
   uint32_t *vals = nullptr, *vals_s = nullptr;
   HD_TRY(ContextArray(ctx, "lbvh_keys", n, &keys));

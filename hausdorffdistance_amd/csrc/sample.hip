@@ -108,8 +108,11 @@ int SampledEbIterations(hd_context* ctx, const DirectedArgs& args,
   const float rate = args.sample_rate > 0.f ? args.sample_rate : 0.01f;
   const uint32_t n_s = (uint32_t)std::min<double>(
       n_a, std::max(1.0, std::ceil((double)n_a * (double)rate)));
+  // B in the pseudo-random order: the iteration counts are compared with
+  // the reference's threshold, which its shuffled order calibrates.
   EbTarget t;
-  HD_TRY(EbPrepareTarget(ctx, nd, args.b, n_b, true, args.seed, &t));
+  HD_TRY(EbPrepareTarget(ctx, nd, args.b, n_b, HD_FLAG_B_ORDER_RANDOM,
+                         args.seed, nullptr, &t));
   float *q = nullptr, *d2 = nullptr, *cm = nullptr;
   uint32_t *nn = nullptr, *counts = nullptr;
   HD_TRY(ContextArray(ctx, "smp_q", (uint64_t)n_s * nd, &q));

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void RadixScatterKernel(
     typename Codec::Stored* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out, uint32_t n, int shift, uint32_t mask,
     uint32_t n_tiles, const uint32_t* __restrict__ counts,
-    const uint32_t* __restrict__ totals) {
+    const uint32_t* __restrict__ totals, uint32_t val_mul) {
   __shared__ uint32_t s_key[kTile];
   __shared__ uint32_t s_val[kTile];
   __shared__ uint32_t wcnt[4][kRadix];
@@ -227,8 +227,8 @@ __global__ __launch_bounds__(kBlock) void RadixScatterKernel(
       const uint32_t k = s_key[i];
       const uint32_t d = (k >> shift) & mask;
       const uint32_t o = out_first[d] + (i - tile_first[d]);
-      keys_out[o] = Codec::Decode(k);
-      vals_out[o] = s_val[i];
+      if (keys_out != nullptr) keys_out[o] = Codec::Decode(k);
+      vals_out[val_mul ? (uint32_t)(((uint64_t)o * val_mul) % n) : o] = s_val[i];
     }
   }
 }
@@ -237,11 +237,12 @@ template <class Codec>
 int RadixSortPairs(hd_context* ctx, const char* what,
                    const typename Codec::Stored* k_in,
                    typename Codec::Stored* k_out, const uint32_t* v_in,
-                   uint32_t* v_out, uint32_t n, int end_bit, int begin_bit = 0) {
+                   uint32_t* v_out, uint32_t n, int end_bit, int begin_bit = 0,
+                   uint32_t out_mul = 0) {
   using Stored = typename Codec::Stored;
   if (n == 0) return HD_OK;
   if (end_bit < 0 || end_bit > 32 || begin_bit < 0 || begin_bit > end_bit ||
-      (k_in == k_out) || (v_in == v_out))
+      (k_in == k_out) || (v_in == v_out) || (out_mul && !k_out))
     return Fail(HD_EINVAL, "%s: bad arguments (bits %d..%d)", what, begin_bit,
                 end_bit);
   const int passes = std::max(1, (end_bit - begin_bit + 7) / 8);
@@ -270,8 +271,12 @@ int RadixSortPairs(hd_context* ctx, const char* what,
         src_k, n, shift, mask, n_tiles, counts);
     RadixScanKernel<<<kRadix, kBlock, 0, ctx->stream>>>(counts, n_tiles,
                                                          totals);
+    // With out_mul, the last pass writes value k to position k * out_mul
+    // mod n of v_out, and no keys.
+    const bool last = p + 1 == passes;
     RadixScatterKernel<Codec><<<n_tiles, kBlock, 0, ctx->stream>>>(
-        src_k, src_v, dst_k, dst_v, n, shift, mask, n_tiles, counts, totals);
+        src_k, src_v, last && out_mul ? nullptr : dst_k, dst_v, n, shift, mask,
+        n_tiles, counts, totals, last ? out_mul : 0u);
     HD_HIP_TRY(hipGetLastError());
     src_k = dst_k;
     src_v = dst_v;
@@ -295,9 +300,9 @@ int SortPairsU16(hd_context* ctx, const uint16_t* k_in, uint16_t* k_out,
 
 int SortPairsU32(hd_context* ctx, const uint32_t* k_in, uint32_t* k_out,
                  const uint32_t* v_in, uint32_t* v_out, uint32_t n,
-                 int end_bit, int begin_bit) {
+                 int end_bit, int begin_bit, uint32_t out_mul) {
   return RadixSortPairs<CodecU32>(ctx, "sort<u32>", k_in, k_out, v_in, v_out,
-                                  n, end_bit, begin_bit);
+                                  n, end_bit, begin_bit, out_mul);
 }
 
 }  // namespace hd

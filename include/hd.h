@@ -88,6 +88,18 @@ extern "C" {
                                         rounds or matrix-core filter. EB
                                         variant only: a comparison
                                         baseline, same result */
+#define HD_FLAG_B_ORDER_RANDOM 0x80u /* early break: visit B in a pseudo-
+                                        random order, as the reference
+                                        shuffles it (early_break.h:51-52,
+                                        132-137) */
+#define HD_FLAG_B_ORDER_STRATIFIED 0x100u /* early break: visit B in a
+                                        spatially stratified order (Morton
+                                        ranks spread evenly over the
+                                        sequence). Neither B_ORDER flag: the
+                                        learned order (HD_FLAG_LEARNED_ORDER)
+                                        takes the stratified one, the other
+                                        schedules the pseudo-random one.
+                                        Same result in every order */
 
 /* hd_run_config.auto_tune (reference -auto_tune, -auto_tune_n_points_cell,
  * -auto_tune_max_hit, src/flags.cc:28-31; the reference predicts them with

@@ -1,0 +1,129 @@
+"""Which visit order of B lets the early break stop soonest? (analysis script
+beside the oracle; test infrastructure, not shipped).
+
+The early break (reference early_break.h:44-115) scans B in one fixed order
+and stops an A point at the first B point closer than sqrt(cmax2). With the
+final cmax2 known, a point's cost is the distance along the order from where
+its scan starts to the first of its "near" B points. This model measures,
+for a sample of A points of the clustered pair:
+
+* mean: the cost of one scan from a random start (the budgeted rounds);
+* seg16: the cost when B is cut into 16 contiguous segments scanned side by
+  side, which stop together at the first hit in any of them (the tail's
+  segmented scans), i.e. 16 x the smallest offset of a near point within
+  its segment;
+
+for these orders:
+
+* random: a pseudo-random permutation (the reference shuffles B);
+* stratified: B ranked by the top 16 bits of its Morton code (the library's
+  frame: 32-bit quantisation per axis over B's MBR, equal codes in input
+  order), rank r at position r * step mod n, step coprime with n and near
+  n / phi (the library's StratifiedOrder);
+* bitrev: the same ranks visited in bit-reversed order (even spacing of
+  aligned blocks, but aligned with power-of-two segments);
+* inverted: position j holding rank j * step mod n (the rotation's map the
+  wrong way round).
+
+Run: python oracle/b_order_model.py [--n 1000000] [--sample 20000]
+(needs scipy for the near sets).
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path[:] = [p for p in sys.path if Path(p or ".").resolve() != HERE]
+sys.path.insert(0, str(HERE.parent))
+
+import hausdorffdistance_amd.gen as gen  # noqa: E402
+
+PHI_INV = 0.6180339887498949
+
+
+def morton_ranks(b: np.ndarray, key_bits: int = 16) -> np.ndarray:
+    """Indices of b sorted by the top key_bits of their 32-bit 2D Morton
+    code in b's MBR (stable)."""
+    lo = b.min(0)
+    hi = b.max(0)
+    levels = 4294967040.0
+    q = np.empty(b.shape, np.uint64)
+    for d in range(2):
+        ext = float(hi[d]) - float(lo[d])
+        scale = np.float32(levels / ext) if ext > 0 else np.float32(0)
+        t = ((b[:, d] - lo[d]).astype(np.float32) * scale).astype(np.float32)
+        t = np.minimum(np.maximum(t, np.float32(0)), np.float32(levels))
+        q[:, d] = t.astype(np.float64).astype(np.uint64)
+
+    def spread(v):
+        x = v.astype(np.uint64)
+        for sh, m in ((16, 0x0000FFFF0000FFFF), (8, 0x00FF00FF00FF00FF),
+                      (4, 0x0F0F0F0F0F0F0F0F), (2, 0x3333333333333333),
+                      (1, 0x5555555555555555)):
+            x = (x | (x << np.uint64(sh))) & np.uint64(m)
+        return x
+
+    code32 = ((spread(q[:, 0]) | (spread(q[:, 1]) << np.uint64(1))) >> np.uint64(32))
+    return np.argsort(code32 >> np.uint64(32 - key_bits), kind="stable")
+
+
+def coprime_step(n: int) -> int:
+    s = max(1, round(n * PHI_INV))
+    while np.gcd(s, n) != 1:
+        s += 1
+    return s
+
+
+def orders(b: np.ndarray, rng) -> dict[str, np.ndarray]:
+    """Visit orders as arrays: order[j] = index of the B point visited j-th."""
+    n = b.shape[0]
+    ranked = morton_ranks(b)
+    step = coprime_step(n)
+    out = {"random": rng.permutation(n)}
+    strat = np.empty(n, np.int64)
+    strat[(np.arange(n, dtype=np.int64) * step) % n] = ranked
+    out["stratified"] = strat
+    bits = max(1, (n - 1).bit_length())
+    p = np.arange(1 << bits, dtype=np.uint64)
+    r = np.zeros_like(p)
+    for k in range(bits):
+        r |= ((p >> np.uint64(k)) & np.uint64(1)) << np.uint64(bits - 1 - k)
+    out["bitrev"] = ranked[r[r < n].astype(np.int64)]
+    out["inverted"] = ranked[(np.arange(n, dtype=np.int64) * step) % n]
+    return out
+
+
+def main() -> None:
+    from scipy.spatial import cKDTree
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--sample", type=int, default=20_000)
+    args = ap.parse_args()
+    n = args.n
+    a, b = gen.gmm_pair(n)
+    rng = np.random.default_rng(1)
+    tree = cKDTree(b.astype(np.float64))
+    d, _ = tree.query(a.astype(np.float64))
+    cmax2 = float(np.max(d) ** 2)
+    sample = rng.choice(n, min(n, args.sample), replace=False)
+    near = tree.query_ball_point(a[sample].astype(np.float64), np.sqrt(cmax2) * (1 - 1e-9))
+    starts = rng.integers(0, n, len(sample))
+    seg = -(-n // 16)
+    print(f"|A| = |B| = {n}, final cmax2 = {cmax2:.6g}, sample of {len(sample)} A points")
+    for name, o in orders(b, rng).items():
+        pos = np.empty(n, np.int64)
+        pos[o] = np.arange(n)
+        mean = np.mean([((pos[np.asarray(nb)] - s) % n).min() + 1 if nb else n
+                        for nb, s in zip(near, starts)])
+        seg16 = np.mean([16 * ((pos[np.asarray(nb)] % seg).min() + 1) if nb else n
+                         for nb in near])
+        print(f"{name:10s} mean {mean:9.1f}  seg16 {seg16:9.1f} evaluations per A point")
+
+
+if __name__ == "__main__":
+    main()
