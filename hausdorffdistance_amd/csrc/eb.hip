@@ -1791,8 +1791,14 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
     // (plain-order mean over 10 runs: 19.8 ms with 256, 16.8 ms with 1024).
     // Without top completions (a learned order whose seed already reached
     // the final cmax2) every escalation joins the next round: completing
-    // 256 entries apart runs at a fifth of a round's evaluation rate.
-    const uint32_t k_top = !complete_top ? 0u
+    // 256 entries apart runs at a fifth of a round's evaluation rate. A
+    // seed that missed the final cmax2 shows as a round escalating more
+    // than an eighth of A (10M GMM: 3.5% after round 2; 20M x 15M sphere,
+    // whose HD comes from radial noise the network does not resolve: 75%,
+    // at 0.31 of the final cmax2), and then the rounds complete their top
+    // entries as in the ranked pipeline.
+    const bool seed_missed = (uint64_t)n_esc * 8u > (uint64_t)n_a;
+    const uint32_t k_top = !complete_top && !seed_missed ? 0u
                            : std::min<uint32_t>(n_esc, (r == 0 && !first_budget) ? 1024u : 256u);
     rs->top_points += k_top;
     bool top_filt = false;

@@ -161,3 +161,26 @@ def test_learned_order_evaluates_fewer_pairs(ctx):
     assert got["learned_single"] < 0.8 * got["plain_single"]
     assert got["learned"] < 0.7 * got["plain"]
     model.close()
+
+
+def test_learned_order_on_a_3d_shell_keeps_bits(ctx):
+    """A pair whose HD comes from radial noise of a thin spherical shell,
+    which the network's inputs barely resolve: its predicted farthest points
+    need not reach the final cmax2, and the rounds then complete their top
+    entries (a round escalating more than an eighth of A). Bits, arg-max
+    and NN stay the oracle's, with the learned order on the early break and
+    on the index variants."""
+    import torch
+
+    a = hd.gen.sphere(31, 200_000, 0.005, 0.0)
+    b = hd.gen.sphere(32, 150_000, 0.005, 2.0)
+    o = oracle.eb_directed(a, b)
+    want = (o.cmax2_bits, o.argmax, o.nn)
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    model = hd.Model.train(ctx, da, db, steps=500, batch=4096, n_samples=100_000)
+    for variant in (hd.Variant.EB, hd.Variant.GRID, hd.Variant.LBVH):
+        r = hd.directed_device(ctx, variant, da, db, learned_order=True, model=model)
+        assert (r.cmax2_bits, r.argmax, r.nn) == want, variant
+    r = hd.directed_device(ctx, hd.Variant.EB, da, db)
+    assert (r.cmax2_bits, r.argmax, r.nn) == want
+    model.close()
