@@ -61,3 +61,20 @@ def test_cli_ply_normalize_translate(tmp_path):
     got = run_cli("-input1", fa, "-input2", fb, "-input_type", "ply", "-n_dims", "3",
                   "-variant", "eb", "-repeat", "1", "-normalize", "-translate", "0.1")
     assert got == (o.cmax2_bits, o.argmax, o.nn)
+
+
+def test_cli_b_order(tmp_path):
+    """-b_order random|stratified|auto selects the early break's visit order
+    of B; every order gives the oracle's bits, an unknown one is refused."""
+    a = hd.gen.uniform(23, 5000, 2).astype(np.float64)
+    b = hd.gen.uniform(24, 4000, 2).astype(np.float64)
+    fa, fb = wkt_file(tmp_path / "a.wkt", a), wkt_file(tmp_path / "b.wkt", b)
+    o = oracle.eb_directed(hd.load_points(fa, "wkt", 2), hd.load_points(fb, "wkt", 2))
+    for order in ("random", "stratified", "auto"):
+        got = run_cli("-input1", fa, "-input2", fb, "-input_type", "wkt", "-n_dims", "2",
+                      "-variant", "eb", "-repeat", "1", "-nocheck", "-b_order", order)
+        assert got == (o.cmax2_bits, o.argmax, o.nn), order
+    bad = subprocess.run([str(EXE), "-input1", fa, "-input2", fb, "-input_type", "wkt",
+                          "-n_dims", "2", "-b_order", "sorted"],
+                         capture_output=True, text=True, timeout=120)
+    assert bad.returncode != 0 and "b_order" in bad.stderr
