@@ -463,12 +463,15 @@ class HausdorffDistanceNearestNeighborSearch(HausdorffDistance):
     algorithm = "Nearest Neighbor Search"
 
 
-def batch_directed(ctx: Context, points_a, offs_a, points_b, offs_b) -> "BatchResults":
+def batch_directed(ctx: Context, points_a, offs_a, points_b, offs_b, *,
+                   shuffle: bool = True) -> "BatchResults":
     """Directed h(A_p -> B_p) for every pair p of a batch (CUDA tensors:
     points (N, d) float32, offsets (n_pairs + 1,) int64 in points). One
-    persistent-kernel launch replayed from a cached HIP graph. Returns a
-    BatchResults: result p is a DirectedResult, the fields of all pairs are
-    numpy arrays."""
+    persistent-kernel launch replayed from a cached HIP graph. shuffle=False
+    scans each pair's B in input order (else its groups of 8 consecutive
+    points in a golden-ratio rotation). Returns a BatchResults: result p is
+    a DirectedResult, the
+    fields of all pairs are numpy arrays."""
     nd = int(points_a.shape[1])
     pa, n_pts_a, da, _ = _points(points_a, nd)
     pb, n_pts_b, db, _ = _points(points_b, nd)
@@ -491,7 +494,7 @@ def batch_directed(ctx: Context, points_a, offs_a, points_b, offs_b) -> "BatchRe
     out = (_lib.DirectedResultC * max(n_pairs, 1))()
     _lib.check(_lib.load().hd_batch_directed_device(
         ctx.handle, nd, pa, C.c_void_p(offs_a.data_ptr()), pb, C.c_void_p(offs_b.data_ptr()),
-        n_pairs, 0, out))
+        n_pairs, 0 if shuffle else _lib.HD_FLAG_NO_SHUFFLE, out))
     return BatchResults(out, n_pairs)
 
 

@@ -45,7 +45,12 @@ struct BatchPair {
   uint32_t n_b;
   uint32_t a_aos;    // offset of the pair's A
   uint32_t n_a;
+  uint32_t b_mul;    // visit order of B: groups of kBGroup, see BuildPlan
 };
+
+// B's visit order moves groups of kBGroup consecutive input points (one
+// 64-byte line of 2D points), so the repack reads whole lines.
+constexpr uint32_t kBGroup = 8;
 
 struct BatchArgs {
   const float* a;
@@ -264,8 +269,13 @@ __global__ void BatchRepackKernel(const float* __restrict__ b,
   for (uint32_t p = blockIdx.x; p < n_pairs; p += gridDim.x) {
     const BatchPair pr = pairs[p];
     const uint32_t n_pad = pr.n_steps * 64u;
+    const uint32_t n = pr.n_b, n_g = n / kBGroup;
     for (uint32_t j = threadIdx.x; j < n_pad; j += blockDim.x) {
-      const uint32_t src = pr.b_aos + (j < pr.n_b ? j : j % pr.n_b);
+      uint32_t k = j < n ? j : j % n;
+      if (k < n_g * kBGroup)
+        k = (uint32_t)(((uint64_t)(k / kBGroup) * pr.b_mul) % n_g) * kBGroup +
+            k % kBGroup;
+      const uint32_t src = pr.b_aos + k;
       const float* q = b + (uint64_t)src * ND;
       x[pr.b_soa + j] = q[0];
       y[pr.b_soa + j] = q[1];
@@ -316,10 +326,18 @@ void DestroyBatchPlan(BatchPlan* p) {
 namespace {
 
 // oa, ob: the offsets (host copies, n_pairs + 1 each).
+// shuffle: each pair's B is visited in groups of kBGroup consecutive input
+// points, group g of the n_g = n_b / kBGroup whole groups at position
+// g * GoldenStep(n_g) mod n_g (the last n_b mod kBGroup points stay last):
+// input that is stored in spatial order (outlines, sorted files) is spread
+// over the scan, so a scan starting anywhere meets a point's neighbours
+// soon; input order otherwise. (Rotating single points instead reads B at
+// random: +0.4 ms on the 4096-pair batch, whose synthetic points are in
+// random order already.)
 template <int ND>
 int BuildPlan(hd_context* ctx, const float* d_a, const std::vector<uint64_t>& oa,
               const float* d_b, const std::vector<uint64_t>& ob,
-              uint32_t n_pairs, uint32_t chunk, BatchPlan* plan) {
+              uint32_t n_pairs, uint32_t chunk, bool shuffle, BatchPlan* plan) {
   std::vector<BatchPair> pairs(n_pairs);
   uint64_t soa = 0;
   for (uint32_t p = 0; p < n_pairs; ++p) {
@@ -336,6 +354,7 @@ int BuildPlan(hd_context* ctx, const float* d_a, const std::vector<uint64_t>& oa
     pr.n_b = (uint32_t)nb;
     pr.a_aos = (uint32_t)oa[p];
     pr.n_a = (uint32_t)na;
+    pr.b_mul = shuffle ? GoldenStep((uint32_t)nb / kBGroup) : 1u;
     soa += (uint64_t)pr.n_steps * 64;
     if (soa > 0xFFFFFFFFull) return Fail(HD_EINVAL, "padded batch too large");
   }
@@ -440,11 +459,12 @@ extern "C" int hd_batch_directed_device(hd_context* ctx, int n_dims,
   if (n_dims != 2 && n_dims != 3) return Fail(HD_EINVAL, "n_dims must be 2 or 3");
   if (n_pairs == 0) return Fail(HD_EINVAL, "no pairs");
   if (flags & ~(HD_FLAG_COUNT_EVALS | HD_FLAG_NO_SHUFFLE))
-    return Fail(HD_EUNSUPPORTED, "batch mode takes COUNT_EVALS only");
+    return Fail(HD_EUNSUPPORTED, "batch mode takes COUNT_EVALS and NO_SHUFFLE only");
   HD_TRY(ContextActivate(ctx));
   char sig[192];
-  snprintf(sig, sizeof(sig), "%d:%p:%p:%u", n_dims, (const void*)d_a,
-           (const void*)d_b, n_pairs);
+  const bool shuffle = (flags & HD_FLAG_NO_SHUFFLE) == 0;
+  snprintf(sig, sizeof(sig), "%d:%p:%p:%u:%d", n_dims, (const void*)d_a,
+           (const void*)d_b, n_pairs, shuffle ? 1 : 0);
   // The offsets are read on every call (2 x (n_pairs + 1) x 8 bytes): the
   // cached plan is reused only for the same point arrays and the same split.
   std::vector<uint64_t> oa(n_pairs + 1), ob(n_pairs + 1);
@@ -471,9 +491,10 @@ extern "C" int hd_batch_directed_device(hd_context* ctx, int n_dims,
       p->sig = sig;
       p->offs_a = oa;
       p->offs_b = ob;
-      const int rc = n_dims == 2
-                         ? BuildPlan<2>(ctx, d_a, oa, d_b, ob, n_pairs, chunk, p)
-                         : BuildPlan<3>(ctx, d_a, oa, d_b, ob, n_pairs, chunk, p);
+      const int rc =
+          n_dims == 2
+              ? BuildPlan<2>(ctx, d_a, oa, d_b, ob, n_pairs, chunk, shuffle, p)
+              : BuildPlan<3>(ctx, d_a, oa, d_b, ob, n_pairs, chunk, shuffle, p);
       if (rc != HD_OK) {
         DestroyBatchPlan(p);
         return rc;

@@ -270,6 +270,21 @@ int FillU64(hd_context* ctx, unsigned long long* p, unsigned long long value,
   return HD_OK;
 }
 
+uint32_t GoldenStep(uint32_t n) {
+  if (n <= 2) return 1u;
+  uint64_t step = (uint64_t)std::llround((double)n * 0.6180339887498949);
+  auto gcd = [](uint64_t x, uint64_t y) {
+    while (y) {
+      const uint64_t t = x % y;
+      x = y;
+      y = t;
+    }
+    return x;
+  };
+  while (gcd(step, n) != 1) ++step;  // ends below n: gcd(n - 1, n) = 1
+  return (uint32_t)step;
+}
+
 int GatherSoA(hd_context* ctx, int n_dims, const float* aos, uint32_t n,
               uint32_t n_pad, bool shuffle, const Feistel& perm, float* x,
               float* y, float* z, const uint32_t* idx) {

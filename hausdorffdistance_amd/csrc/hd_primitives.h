@@ -40,6 +40,12 @@ int FillU32(hd_context* ctx, uint32_t* p, uint32_t value, uint64_t n);
 int FillU64(hd_context* ctx, unsigned long long* p, unsigned long long value,
             uint64_t n);
 
+// A step coprime with n near n / phi (1 for n <= 2): i -> i * step mod n
+// is a bijection of [0, n) that spreads every run of consecutive i evenly
+// over [0, n) (the gaps of an irrational rotation take at most three
+// values).
+uint32_t GoldenStep(uint32_t n);
+
 // Repacks AoS points into SoA arrays of n_pad entries (n_pad >= n),
 // visiting source index idx[j mod n] for destination j when idx (a device
 // array of n indices) is given, else perm(j mod n) (identity if !shuffle).

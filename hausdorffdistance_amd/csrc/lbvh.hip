@@ -508,26 +508,15 @@ int StratifiedOrder(hd_context* ctx, int n_dims, const float* pts, uint32_t n,
     std::memcpy(m, mbr, sizeof(float) * 2 * n_dims);
   else
     HD_TRY(HostMbr(ctx, n_dims, pts, n, m));
-  // Rank r goes to position r * step mod n, step coprime with n and near
-  // n / phi: the ranks of any run (the points of one region) land evenly
-  // spread over the sequence (the gaps of an irrational rotation take at
-  // most three values).
-  uint64_t step = std::max<uint64_t>(1, (uint64_t)std::llround((double)n * 0.6180339887498949));
-  auto gcd = [](uint64_t x, uint64_t y) {
-    while (y) {
-      const uint64_t t = x % y;
-      x = y;
-      y = t;
-    }
-    return x;
-  };
-  while (n > 1 && gcd(step, n) != 1) ++step;
+  // Rank r goes to position r * step mod n (GoldenStep): the ranks of any
+  // run (the points of one region) land evenly spread over the sequence.
+  const uint32_t step = GoldenStep(n);
   // Ranks by the top 16 bits of the Morton code (cells of 2^-8 of the
   // extent per axis in 2D, about 2^-5 in 3D; two radix passes: a CPU model
   // of the 1M GMM pair gives the same evaluations as with 24 bits); the
   // sort's last pass writes each index straight to its position.
   HD_TRY(MortonOrder(ctx, n_dims, pts, n, MortonFrame(n_dims, m), perm_out,
-                     16, "border_", n > 1 ? (uint32_t)step : 0u));
+                     16, "border_", n > 1 ? step : 0u));
   return HD_OK;
 }
 

@@ -269,9 +269,13 @@ int hd_nn_device(hd_context* ctx, int index, int n_dims, const float* d_q,
 /* Batched directed HD: pair p is A[offs_a[p] .. offs_a[p+1]) against
  * B[offs_b[p] .. offs_b[p+1]), all device-resident (AoS, offsets in points,
  * n_pairs + 1 entries each). One persistent-kernel launch captured in a HIP
- * graph; tiles come from the autotuner. out: host array of n_pairs results
- * (evals and time_ms are per batch, reported in out[0]). No counterpart in
- * the reference, which runs one pair per process. */
+ * graph; tiles come from the autotuner. flags: HD_FLAG_COUNT_EVALS,
+ * HD_FLAG_NO_SHUFFLE (each pair's B scanned in input order; by default its
+ * groups of 8 consecutive points in the order g -> g * step mod n_g, n_g =
+ * |B_p| / 8 whole groups, step coprime with n_g near n_g / phi, the last
+ * |B_p| mod 8 points last). out: host array of n_pairs results (evals and time_ms are
+ * per batch, reported in out[0]). No counterpart in the reference, which
+ * runs one pair per process. */
 int hd_batch_directed_device(hd_context* ctx, int n_dims, const float* d_a,
                              const uint64_t* d_offs_a, const float* d_b,
                              const uint64_t* d_offs_b, uint32_t n_pairs,

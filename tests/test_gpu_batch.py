@@ -96,3 +96,33 @@ def test_batch_offsets_are_validated(ctx):
                           b, good_b)
     res = hd.batch_directed(ctx, a, good_a, b, good_b)
     assert len(res) == 2
+
+
+def test_batch_b_order_on_spatially_sorted_input(ctx):
+    """Pairs whose B is stored sorted along x (as polygon outlines or sorted
+    files are): both visit orders give the oracle's bits; in input order a
+    scan meets a point's neighbours only when it reaches their stretch of
+    the array, the default rotation of B's indices spreads them over the
+    scan."""
+    import torch
+
+    bt = hd.gen.batch(19, 48)
+    b = bt.b.copy()
+    for p in range(len(bt.offs_b) - 1):
+        s, e = bt.offs_b[p], bt.offs_b[p + 1]
+        b[s:e] = b[s:e][np.argsort(b[s:e, 0], kind="stable")]
+    da = torch.from_numpy(np.ascontiguousarray(bt.a)).cuda()
+    db = torch.from_numpy(np.ascontiguousarray(b)).cuda()
+    doa = torch.from_numpy(bt.offs_a.astype(np.int64)).cuda()
+    dob = torch.from_numpy(bt.offs_b.astype(np.int64)).cuda()
+    got = {}
+    for shuffle in (True, False):
+        r = hd.batch_directed(ctx, da, doa, db, dob, shuffle=shuffle)
+        for p in range(len(bt.offs_a) - 1):
+            o = oracle.eb_directed(bt.a[bt.offs_a[p]:bt.offs_a[p + 1]],
+                                   b[bt.offs_b[p]:bt.offs_b[p + 1]], n_threads=4)
+            assert (r[p].cmax2_bits, r[p].argmax, r[p].nn) == \
+                (o.cmax2_bits, o.argmax, o.nn), (shuffle, p)
+        got[shuffle] = r[0].evals
+    print(f"[batch] evaluations, rotated / input order: {got[True]} / {got[False]}", flush=True)
+    assert got[True] < 0.8 * got[False]
