@@ -152,7 +152,8 @@ __global__ __launch_bounds__(kBlock) void RadixScatterKernel(
     typename Codec::Stored* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out, uint32_t n, int shift, uint32_t mask,
     uint32_t n_tiles, const uint32_t* __restrict__ counts,
-    const uint32_t* __restrict__ totals, uint32_t val_mul) {
+    const uint32_t* __restrict__ totals, uint32_t val_mul,
+    unsigned long long val_magic) {
   __shared__ uint32_t s_key[kTile];
   __shared__ uint32_t s_val[kTile];
   __shared__ uint32_t wcnt[4][kRadix];
@@ -228,7 +229,17 @@ __global__ __launch_bounds__(kBlock) void RadixScatterKernel(
       const uint32_t d = (k >> shift) & mask;
       const uint32_t o = out_first[d] + (i - tile_first[d]);
       if (keys_out != nullptr) keys_out[o] = Codec::Decode(k);
-      vals_out[val_mul ? (uint32_t)(((uint64_t)o * val_mul) % n) : o] = s_val[i];
+      uint32_t dst = o;
+      if (val_mul) {
+        // o * val_mul mod n: q = mulhi(x, floor((2^64 - 1) / n)) is
+        // floor(x / n) or one less (x < 2^62), so one subtraction fixes r.
+        const unsigned long long x = (unsigned long long)o * val_mul;
+        const unsigned long long q = __umul64hi(x, val_magic);
+        unsigned long long r = x - q * n;
+        if (r >= n) r -= n;
+        dst = (uint32_t)r;
+      }
+      vals_out[dst] = s_val[i];
     }
   }
 }
@@ -276,7 +287,7 @@ int RadixSortPairs(hd_context* ctx, const char* what,
     const bool last = p + 1 == passes;
     RadixScatterKernel<Codec><<<n_tiles, kBlock, 0, ctx->stream>>>(
         src_k, src_v, last && out_mul ? nullptr : dst_k, dst_v, n, shift, mask,
-        n_tiles, counts, totals, last ? out_mul : 0u);
+        n_tiles, counts, totals, last ? out_mul : 0u, ~0ull / n);
     HD_HIP_TRY(hipGetLastError());
     src_k = dst_k;
     src_v = dst_v;
