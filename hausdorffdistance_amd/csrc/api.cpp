@@ -87,6 +87,13 @@ int AcquireContext(hd_context* given, ScopedContext* sc) {
 
 bool ValidVariant(int v) { return v >= HD_VARIANT_EB && v <= HD_VARIANT_RT_HDIST; }
 
+// The two visit orders of B exclude each other.
+int ValidateBOrder(uint32_t flags) {
+  if ((flags & HD_FLAG_B_ORDER_RANDOM) && (flags & HD_FLAG_B_ORDER_STRATIFIED))
+    return Fail(HD_EINVAL, "B_ORDER_RANDOM and B_ORDER_STRATIFIED exclude each other");
+  return HD_OK;
+}
+
 int ValidateSizes(uint64_t n_a, uint64_t n_b) {
   if (n_a == 0 || n_b == 0)
     return Fail(HD_EINVAL, "empty point set (|A|=%llu, |B|=%llu)",
@@ -566,6 +573,7 @@ int hd_directed_device(hd_context* ctx, int variant, int n_dims,
     return Fail(HD_EINVAL, "unknown variant %d", variant);
   if ((flags & HD_FLAG_SINGLE_PASS) && variant != HD_VARIANT_EB)
     return Fail(HD_EINVAL, "SINGLE_PASS applies to the early-break variant");
+  HD_TRY(ValidateBOrder(flags));
   HD_TRY(ValidateSizes(n_a, n_b));
   if (!d_a || !d_b) return Fail(HD_EINVAL, "null point array");
   ScopedContext sc;
@@ -608,6 +616,7 @@ int hd_run_hausdorff_distance(hd_context* ctx, const hd_run_config* cfg,
     return Fail(HD_EINVAL, "Unknown execution: %d", cfg->execution);
   if (cfg->repeat < 1)
     return Fail(HD_EINVAL, "repeat must be >= 1, got %d", cfg->repeat);
+  HD_TRY(ValidateBOrder(cfg->flags));
   HD_TRY(ValidateSizes(n_a, n_b));
   if (!a || !b) return Fail(HD_EINVAL, "null point array");
   if (cfg->execution == HD_EXEC_CPU)

@@ -99,7 +99,9 @@ extern "C" {
                                         learned order (HD_FLAG_LEARNED_ORDER)
                                         takes the stratified one, the other
                                         schedules the pseudo-random one.
-                                        Same result in every order */
+                                        The two B_ORDER flags exclude each
+                                        other (HD_EINVAL). Same result in
+                                        every order */
 
 /* hd_run_config.auto_tune (reference -auto_tune, -auto_tune_n_points_cell,
  * -auto_tune_max_hit, src/flags.cc:28-31; the reference predicts them with

@@ -140,3 +140,32 @@ def test_cli_cpu_execution_checks_by_default():
     win = max((full["ab"], 0), (full["ba"], 1), key=lambda t: t[0]["cmax2_bits"])
     assert (int(m.group(1), 16), int(m.group(2)), int(m.group(3)), int(m.group(4))) == (
         win[0]["cmax2_bits"], win[0]["argmax"], win[0]["nn"], win[1])
+
+
+def test_b_order_flags_exclude_each_other():
+    """HD_FLAG_B_ORDER_RANDOM with HD_FLAG_B_ORDER_STRATIFIED is refused at
+    the boundary (checked before the execution is chosen, so on the CPU
+    too); one of them alone runs, and the Python interface refuses an
+    unknown order name."""
+    import ctypes as C
+
+    from hausdorffdistance_amd import _lib
+
+    a, b = hd.gen.uniform_pair(200, 2)
+    lib = _lib.load()
+    pa = a.ctypes.data_as(C.POINTER(C.c_float))
+    pb = b.ctypes.data_as(C.POINTER(C.c_float))
+    for extra, ok in ((_lib.HD_FLAG_B_ORDER_RANDOM, True),
+                      (_lib.HD_FLAG_B_ORDER_STRATIFIED, True),
+                      (_lib.HD_FLAG_B_ORDER_RANDOM | _lib.HD_FLAG_B_ORDER_STRATIFIED, False)):
+        c = hd.RunConfig(execution=hd.Execution.CPU, n_dims=2, repeat=1).to_c()
+        c.flags |= extra
+        out = _lib.ResultC()
+        buf = C.create_string_buffer(1 << 16)
+        rc = lib.hd_run_hausdorff_distance(None, C.byref(c), pa, len(a), pb, len(b),
+                                           C.byref(out), buf, 1 << 16)
+        assert (rc == 0) == ok, (extra, rc, lib.hd_last_error())
+        if not ok:
+            assert rc == _lib.HD_EINVAL and b"exclude" in lib.hd_last_error()
+    with pytest.raises(hd.HDError):
+        hd.RunConfig(b_order="sorted").to_c()
