@@ -269,12 +269,26 @@ __global__ void BatchRepackKernel(const float* __restrict__ b,
   for (uint32_t p = blockIdx.x; p < n_pairs; p += gridDim.x) {
     const BatchPair pr = pairs[p];
     const uint32_t n_pad = pr.n_steps * 64u;
-    const uint32_t n = pr.n_b, n_g = n / kBGroup;
+    const uint32_t n = pr.n_b, n_g = n / kBGroup, whole = n_g * kBGroup;
+    // A thread's slots j advance by blockDim.x (a multiple of kBGroup), so
+    // their group advances by blockDim.x / kBGroup: its rotated position is
+    // kept incrementally (one 64-bit remainder per thread, not per point).
+    const uint32_t inc =
+        n_g ? (uint32_t)(((uint64_t)(blockDim.x / kBGroup) * pr.b_mul) % n_g) : 0u;
+    uint32_t r = n_g ? (uint32_t)(((uint64_t)(threadIdx.x / kBGroup % n_g) *
+                                   pr.b_mul) % n_g)
+                     : 0u;
     for (uint32_t j = threadIdx.x; j < n_pad; j += blockDim.x) {
-      uint32_t k = j < n ? j : j % n;
-      if (k < n_g * kBGroup)
-        k = (uint32_t)(((uint64_t)(k / kBGroup) * pr.b_mul) % n_g) * kBGroup +
-            k % kBGroup;
+      uint32_t k;
+      if (j < whole) {
+        k = r * kBGroup + j % kBGroup;
+        r = r >= n_g - inc ? r - (n_g - inc) : r + inc;
+      } else {  // the last n mod kBGroup points, and the padding
+        k = j < n ? j : j % n;
+        if (k < whole)
+          k = (uint32_t)(((uint64_t)(k / kBGroup) * pr.b_mul) % n_g) * kBGroup +
+              k % kBGroup;
+      }
       const uint32_t src = pr.b_aos + k;
       const float* q = b + (uint64_t)src * ND;
       x[pr.b_soa + j] = q[0];
@@ -420,6 +434,7 @@ int BuildPlan(hd_context* ctx, const float* d_a, const std::vector<uint64_t>& oa
   HD_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   ZeroKernel<<<std::max(1u, std::min(4096u, (n_pairs + 255) / 256)), 256, 0,
                ctx->stream>>>(keys, n_pairs, claim, evals);
+  static_assert(256 % kBGroup == 0, "repack blocks step whole groups");
   BatchRepackKernel<ND><<<std::min(n_pairs, 8192u), 256, 0, ctx->stream>>>(
       d_b, d_pairs, n_pairs, bx, by, bz);
   BatchEbKernel<ND><<<blocks, kBlock, 0, ctx->stream>>>(ba);
