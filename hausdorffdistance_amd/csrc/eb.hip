@@ -1695,9 +1695,10 @@ int ProcessOpen(hd_context* ctx, const float* a, uint32_t n_a,
                 bool complete_top = true) {
   static const uint32_t kRanked[] = {16u, 256u, 1024u};
   // Without top completions (the learned order, cmax2 final from its seed)
-  // longer budgets escalate fewer points (10M GMM: 256/1024 12.6 ms,
-  // 384/1536 12.3, 512/2048 12.5, 128/1024 13.1).
-  static const uint32_t kFinal[] = {16u, 384u, 1536u};
+  // longer budgets escalate fewer points (10M GMM, stratified order of B:
+  // 512/1536 10.1 ms, 384/1536 10.2, 768/1536 10.15, 512/1280 10.2,
+  // 384/1024 10.65, 384/2048 10.8).
+  static const uint32_t kFinal[] = {16u, 512u, 1536u};
   const uint32_t* kBudgets = complete_top ? kRanked : kFinal;
   constexpr int kRounds = (int)(sizeof(kRanked) / sizeof(kRanked[0]));
   static_assert(kRounds <= kMaxRounds, "round statistics");
