@@ -715,7 +715,7 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
     auto next_step = [&](uint32_t st) __attribute__((always_inline)) {
       return st + 1u == seg_end ? seg_begin : st + 1u;
     };
-    float bcur[4 * kFilterSteps];
+    float bcur[4 * kFilterSteps], bnext[4 * kFilterSteps];
     uint32_t nxt = step;  // first step of the next iteration
 #pragma unroll
     for (int k = 0; k < kFilterSteps; ++k) {
@@ -723,7 +723,12 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
       nxt = next_step(nxt);
     }
 
-    while (true) {
+    // Two iterations per pass with the tile buffers' roles swapped: the
+    // prefetched tiles become the current ones with no copy or reload
+    // (10M GMM: learned 10.08 -> 9.99 ms, plain 12.94 -> 12.79 ms).
+    auto iteration = [&](float (&bcur)[4 * kFilterSteps],
+                         float (&bnx)[4 * kFilterSteps])
+        __attribute__((always_inline)) -> bool {
       // Refill free slots from the compacted reservoir.
       unsigned long long free_m = ~live;
       while (free_m != 0ull) {
@@ -821,7 +826,7 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
         free_m &= ~took_m;
         rhead += took;
       }
-      if (live == 0ull) break;
+      if (live == 0ull) return false;
 
       // Segmented scans poll the other segments' verdicts every iteration
       // (the tail and top completions: a segment stops at most 4 steps after
@@ -839,7 +844,6 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
       // B tiles against the 4 A tiles (16 kFilterSteps MFMAs).
       const uint32_t cur_step = step;
       step = nxt;
-      float bnx[4 * kFilterSteps];
   #pragma unroll
       for (int k = 0; k < kFilterSteps; ++k) {
         LoadStepTiles(args.bt, nxt, lane4, bnx + 4 * k);
@@ -990,8 +994,9 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
         if (n_stg == (uint32_t)kWaveSize) flush();
       }
       live &= ~(hit | gone | due);
-  #pragma unroll
-      for (int q = 0; q < 4 * kFilterSteps; ++q) bcur[q] = bnx[q];
+      return true;
+    };
+    while (iteration(bcur, bnext) && iteration(bnext, bcur)) {
     }
   }
   if (n_stg > 0) flush();
