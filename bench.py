@@ -621,13 +621,16 @@ def main() -> None:
             # the pipeline without the network: seed, a ranking round on a
             # sample of the pseudo-random order, top completions, budgeted
             # rounds on the matrix-core filter, segmented tail
-            "plain_is": "ranked pipeline (pseudo-random order, no network)",
+            "plain_is": "ranked pipeline (pseudo-random orders of A and B, no network)",
             "single_pass_ms_per_step": ms_single,
             "single_pass_value": n_a / (ms_single / 1e3),
             "single_pass_spread": spread(res_single),
             "single_pass_is": "one pass over the pseudo-random order, cmax2 from 0, exact "
                               "scan only: the reference GPU kernel's schedule",
             "evals_single_pass": rs.evals,
+            "learned_is": "A in descending predicted NN distance (the network), B in "
+                          "the stratified order (Morton ranks spread by a golden-ratio "
+                          "rotation), seed of 2048, rounds of 512 and 1536 steps, tail",
             "learned_ms_per_step": ms_learned,
             "learned_spread": spread(res_learned) if use_learned else None,
             "learned_speedup": (ms_plain / ms_learned) if use_learned else None,
