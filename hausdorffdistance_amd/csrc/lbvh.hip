@@ -626,10 +626,35 @@ int BuildLbvh(hd_context* ctx, int n_dims, const float* b, uint64_t n_b,
   return HD_OK;
 }
 
+// One-wave blocks of the query kernel that a CU runs at once. The queries
+// are split evenly over the blocks, so a launch of more blocks than run at
+// once finishes with a late partial round. The occupancy query reports 10
+// per CU on gfx950 (its 16 KB LDS stack), but launches behave as if 8 run
+// at once (10M GMM: 8 per CU 2.53 ms, 10 2.77, 12 2.67, 16 2.54; 20M x
+// 15M sphere: 16 per CU 5.29 ms, 10 5.71), so at most 8 are used.
+constexpr int kLbvhBlocksPerCu = 8;
+
+template <int ND>
+int LbvhResidentBlocks(hd_context* ctx) {
+  static int per_cu[4] = {0, 0, 0, 0};
+  if (per_cu[ND] == 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, LbvhQueryKernel<ND>, 64,
+                                                     0) != hipSuccess ||
+        n < 1) {
+      (void)hipGetLastError();
+      n = 1;
+    }
+    per_cu[ND] = std::min(n, kLbvhBlocksPerCu);
+  }
+  return per_cu[ND] * ctx->cu_count;
+}
+
 int LaunchLbvhQuery(hd_context* ctx, const LbvhQueryArgs& args) {
   if (args.pos_begin >= args.pos_end) return HD_OK;
   const uint32_t n = args.pos_end - args.pos_begin;
-  int blocks = ctx->cu_count * 12;
+  int blocks = args.tree_nd == 2 ? LbvhResidentBlocks<2>(ctx)
+                                 : LbvhResidentBlocks<3>(ctx);
   blocks = (int)std::min<uint32_t>((uint32_t)blocks, (n + 63) / 64);
   blocks = std::max(blocks, 1);
   StageDispatch(ctx);
