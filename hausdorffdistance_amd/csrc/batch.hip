@@ -45,7 +45,8 @@ struct BatchPair {
   uint32_t n_b;
   uint32_t a_aos;    // offset of the pair's A
   uint32_t n_a;
-  uint32_t b_mul;    // visit order of B: groups of kBGroup, see BuildPlan
+  uint32_t b_mul;    // visit order of B: slot group g holds input group
+                     // g * b_mul mod n_g (see BuildPlan)
 };
 
 // B's visit order moves groups of kBGroup consecutive input points (one
@@ -339,6 +340,22 @@ void DestroyBatchPlan(BatchPlan* p) {
 
 namespace {
 
+// The inverse of a modulo n (a coprime with n; 1 for n <= 1), by the
+// extended Euclidean algorithm.
+uint32_t ModInverse(uint32_t a, uint32_t n) {
+  if (n <= 1) return 1u;
+  int64_t r0 = n, r1 = a % n, t0 = 0, t1 = 1;
+  while (r1 != 0) {
+    const int64_t q = r0 / r1;
+    const int64_t r2 = r0 - q * r1, t2 = t0 - q * t1;
+    r0 = r1;
+    r1 = r2;
+    t0 = t1;
+    t1 = t2;
+  }
+  return (uint32_t)(t0 < 0 ? t0 + (int64_t)n : t0);
+}
+
 // oa, ob: the offsets (host copies, n_pairs + 1 each).
 // shuffle: each pair's B is visited in groups of kBGroup consecutive input
 // points, group g of the n_g = n_b / kBGroup whole groups at position
@@ -368,7 +385,14 @@ int BuildPlan(hd_context* ctx, const float* d_a, const std::vector<uint64_t>& oa
     pr.n_b = (uint32_t)nb;
     pr.a_aos = (uint32_t)oa[p];
     pr.n_a = (uint32_t)na;
-    pr.b_mul = shuffle ? GoldenStep((uint32_t)nb / kBGroup) : 1u;
+    // Input group g goes to slot group g * step mod n_g, so the repack's
+    // slot group s reads input group s * step^-1 mod n_g. (The map the
+    // other way round, s reading s * step, spreads the input groups of one
+    // region unevenly: a CPU model over 300 group counts puts a scan's
+    // distance to the first of 16 neighbouring groups at up to 7x that of
+    // a random order, against at most 0.8x this way.)
+    const uint32_t n_g = (uint32_t)nb / kBGroup;
+    pr.b_mul = shuffle ? ModInverse(GoldenStep(n_g), n_g) : 1u;
     soa += (uint64_t)pr.n_steps * 64;
     if (soa > 0xFFFFFFFFull) return Fail(HD_EINVAL, "padded batch too large");
   }
