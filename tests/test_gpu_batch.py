@@ -126,3 +126,59 @@ def test_batch_b_order_on_spatially_sorted_input(ctx):
         got[shuffle] = r[0].evals
     print(f"[batch] evaluations, rotated / input order: {got[True]} / {got[False]}", flush=True)
     assert got[True] < 0.8 * got[False]
+
+
+def _lattice_pair(n: int, n_dims: int, rng):
+    """B: one point near the centre of each of n distinct lattice cells
+    (side s, offsets below 0.1/s), in random order; A[i] = B[i] moved by less
+    than 0.01/s per axis, so the nearest neighbour of A[i] is B[i] alone and
+    a B point that the repack drops or duplicates changes h(A->B)."""
+    side = int(np.ceil(n ** (1.0 / n_dims)))
+    while side ** n_dims < n:
+        side += 1
+    cells = rng.permutation(side ** n_dims)[:n]
+    coords = np.stack([(cells // side ** d) % side for d in range(n_dims)], 1)
+    b = ((coords + 0.5 + rng.uniform(-0.1, 0.1, coords.shape)) / side).astype(np.float32)
+    a = (b.astype(np.float64) + rng.uniform(-0.01, 0.01, coords.shape) / side).astype(np.float32)
+    return a, b
+
+
+def _lattice_expected(a: np.ndarray, b: np.ndarray):
+    """h(A->B) of a lattice pair from its construction: FP32 d2[i] with the
+    reference's unfused left-to-right sum; the arg-max is the smallest
+    index of the largest d2, its nearest neighbour B at the same index."""
+    d2 = np.zeros(len(a), np.float32)
+    for k in range(a.shape[1]):
+        t = a[:, k] - b[:, k]
+        d2 = d2 + t * t
+    i = int(np.flatnonzero(d2 == d2.max())[0])
+    return int(d2.view(np.uint32)[i]), i, i
+
+
+@pytest.mark.parametrize("n_dims", [2, 3])
+def test_batch_repack_visits_every_b_point(ctx, n_dims):
+    """The repack's visit orders (groups of 8 in a golden-ratio rotation,
+    and input order) are permutations of each pair's B: sizes below one
+    group, whole groups, remainders, group counts with small factors
+    (1001 = 7 x 11 x 13, 1024) and a prime, each pair against the answer
+    its construction fixes (brute force confirms it on the small pairs)."""
+    import torch
+
+    rng = np.random.default_rng(11 + n_dims)
+    sizes = [1, 2, 7, 8, 9, 15, 16, 17, 64, 100, 257, 1000, 4097,
+             8 * 1001, 8 * 1024, 8 * 3000 + 5, 30011]
+    pairs = [_lattice_pair(n, n_dims, rng) for n in sizes]
+    want = [_lattice_expected(a, b) for a, b in pairs]
+    for (a, b), w in zip(pairs, want):
+        if len(a) <= 1000:
+            o = oracle.brute_directed(a, b)
+            assert (o.cmax2_bits, o.argmax, o.nn) == w
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    da = torch.from_numpy(np.concatenate([p[0] for p in pairs])).cuda()
+    db = torch.from_numpy(np.concatenate([p[1] for p in pairs])).cuda()
+    doffs = torch.from_numpy(offs).cuda()
+    print(f"[batch] lattice {n_dims}D, {len(sizes)} pairs", flush=True)
+    for shuffle in (True, False):
+        r = hd.batch_directed(ctx, da, doffs, db, doffs.clone(), shuffle=shuffle)
+        for p, w in enumerate(want):
+            assert (r[p].cmax2_bits, r[p].argmax, r[p].nn) == w, (shuffle, sizes[p])
