@@ -329,8 +329,9 @@ def directed_device(ctx: Context, variant: Variant, points_a, points_b, *,
     filter, the reference GPU kernel's schedule (a comparison baseline).
     b_order (early break): the visit order of B, "random" (pseudo-random,
     as the reference shuffles it), "stratified" (Morton ranks spread evenly
-    over the sequence) or "auto" (stratified with learned_order, else
-    random); the result is the same in every order."""
+    over the sequence) or "auto" (stratified with learned_order unless
+    single_pass, else random); shuffle=False (input order) overrides it.
+    The result is the same in every order."""
     nd = n_dims or int(points_a.shape[1])
     pa, na, da, _ = _points(points_a, nd)
     pb, nb, db, _ = _points(points_b, nd)

@@ -97,8 +97,12 @@ extern "C" {
                                         ranks spread evenly over the
                                         sequence). Neither B_ORDER flag: the
                                         learned order (HD_FLAG_LEARNED_ORDER)
-                                        takes the stratified one, the other
+                                        takes the stratified one unless it
+                                        runs as a single pass, the other
                                         schedules the pseudo-random one.
+                                        HD_FLAG_NO_SHUFFLE (input order)
+                                        overrides both flags; above 2^31 B
+                                        points the order is pseudo-random.
                                         The two B_ORDER flags exclude each
                                         other (HD_EINVAL). Same result in
                                         every order */
