@@ -62,9 +62,15 @@ HD_DEV uint32_t OrderAt(int mode, const uint32_t* order, const Feistel& perm,
   return order[pos];
 }
 
-// Segment scans. B's steps are split into n_segs segments of seg_steps
-// steps (the last may be shorter). Every segment has its own queue over the
-// same list of order positions; a wave works on one segment at a time and
+// Segment scans. B's steps are dealt to n_segs segments: segment s takes
+// steps s, s + n_segs, s + 2 n_segs, ... (about seg_steps of them), so the
+// segments of an entry advance through B's visit order side by side. (With
+// contiguous ranges instead, the points of one region, which the stratified
+// order spreads evenly, sit at similar offsets in every range, and a hit
+// comes no sooner than in one range: 10M GMM, learned order, 2% more
+// evaluations, tail 2.46 against 2.34 ms.) Every segment has its own queue
+// over the same list of order positions; a wave works on one segment at a
+// time and
 // keeps P slots, each holding one A point in scalar registers. The wave's
 // lanes stream the segment's B steps cyclically (64 coalesced SoA points per
 // step) and every B point in registers is tested against all P slots, so a
@@ -178,8 +184,11 @@ struct ScanWave {
   unsigned long long evals;
 };
 
-HD_DEV uint32_t NextInSeg(uint32_t st, uint32_t seg_begin, uint32_t seg_end) {
-  return st + 1 == seg_end ? seg_begin : st + 1;
+// The step after st in a segment: every seg_stride-th step from seg_begin,
+// cyclic below seg_end.
+HD_DEV uint32_t NextInSeg(uint32_t st, uint32_t seg_begin, uint32_t seg_end,
+                          uint32_t seg_stride) {
+  return st + seg_stride >= seg_end ? seg_begin : st + seg_stride;
 }
 
 // Steps a wave's slots S B steps at a time until a slot breaks (hit), is
@@ -187,19 +196,20 @@ HD_DEV uint32_t NextInSeg(uint32_t st, uint32_t seg_begin, uint32_t seg_end) {
 template <int ND, int P, int S, bool kAllLive>
 HD_DEV void ScanIterate(const ScanArgs& args, ScanWave<ND, P, S>& w,
                         uint32_t lane, uint32_t seg_begin, uint32_t seg_end,
-                        bool split, uint32_t next_due, uint32_t& cmax2,
-                        uint32_t& hit, uint32_t& gone) {
+                        uint32_t seg_stride, bool split, uint32_t next_due,
+                        uint32_t& cmax2, uint32_t& hit, uint32_t& gone) {
   const uint32_t* cm_ptr = reinterpret_cast<const uint32_t*>(args.key) + 1;
   while (true) {
     float bnext[S][ND];
     uint32_t st = w.step;
 #pragma unroll
-    for (int k = 0; k < S; ++k) st = NextInSeg(st, seg_begin, seg_end);
+    for (int k = 0; k < S; ++k)
+      st = NextInSeg(st, seg_begin, seg_end, seg_stride);
     const uint32_t step_next = st;
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       LoadStep<ND>(args.bx, args.by, args.bz, st, lane, bnext[k]);
-      st = NextInSeg(st, seg_begin, seg_end);
+      st = NextInSeg(st, seg_begin, seg_end, seg_stride);
     }
     const uint32_t cm_next =
         __hip_atomic_load(cm_ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -237,9 +247,12 @@ HD_DEV void ScanSegment(const ScanArgs& args, uint32_t seg, uint32_t lane,
                         uint32_t& stg_idx, float& stg_min, uint32_t& n_stg) {
   constexpr uint32_t kAll = ScanWave<ND, P, S>::kAll;
   const bool split = args.n_segs > 1;
-  const uint32_t seg_begin = seg * args.seg_steps;
-  const uint32_t seg_len = min(args.seg_steps, args.n_steps - seg_begin);
-  const uint32_t seg_end = seg_begin + seg_len;
+  // Segment seg: steps seg, seg + n_segs, ... (n_segs == 1: all of B).
+  const uint32_t seg_stride = args.n_segs;
+  const uint32_t seg_begin = seg;
+  const uint32_t seg_len =
+      (args.n_steps - seg + args.n_segs - 1) / args.n_segs;
+  const uint32_t seg_end = args.n_steps;
   const uint32_t limit = split ? seg_len : min(args.budget_steps, seg_len);
   unsigned int* cursor = args.cursors + seg;
 
@@ -276,7 +289,7 @@ HD_DEV void ScanSegment(const ScanArgs& args, uint32_t seg, uint32_t lane,
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       LoadStep<ND>(args.bx, args.by, args.bz, st, lane, w.bcur[k]);
-      st = NextInSeg(st, seg_begin, seg_end);
+      st = NextInSeg(st, seg_begin, seg_end, seg_stride);
     }
   }
   while (true) {
@@ -332,11 +345,13 @@ HD_DEV void ScanSegment(const ScanArgs& args, uint32_t seg, uint32_t lane,
     // branches) and some slots free (the cursor has drained).
     uint32_t hit = 0, gone = 0;
     if (w.live == kAll)
-      ScanIterate<ND, P, S, true>(args, w, lane, seg_begin, seg_end, split,
-                                  next_due, cmax2, hit, gone);
+      ScanIterate<ND, P, S, true>(args, w, lane, seg_begin, seg_end,
+                                  seg_stride, split, next_due, cmax2, hit,
+                                  gone);
     else
-      ScanIterate<ND, P, S, false>(args, w, lane, seg_begin, seg_end, split,
-                                   next_due, cmax2, hit, gone);
+      ScanIterate<ND, P, S, false>(args, w, lane, seg_begin, seg_end,
+                                   seg_stride, split, next_due, cmax2, hit,
+                                   gone);
 
     w.live &= ~(hit | gone);
     if (w.gstep >= next_due) {
@@ -683,9 +698,14 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
   const uint32_t sweep = n_waves > args.n_segs ? n_waves : args.n_segs;
   for (uint32_t kk = wave_id; kk < sweep; kk += n_waves) {
     const uint32_t seg = kk % args.n_segs;
-    const uint32_t seg_begin = seg * args.seg_steps;
-    const uint32_t seg_len = min(args.seg_steps, n_steps - seg_begin);
-    const uint32_t seg_end = seg_begin + seg_len;
+    // Segment seg: steps seg, seg + n_segs, ... (one segment: all of B);
+    // a grouped round's segments are contiguous ranges of its budget.
+    const uint32_t seg_stride = grouped ? 1u : args.n_segs;
+    const uint32_t seg_begin = grouped ? seg * args.seg_steps : seg;
+    const uint32_t seg_len =
+        grouped ? min(args.seg_steps, n_steps - seg_begin)
+                : (n_steps - seg + args.n_segs - 1) / args.n_segs;
+    const uint32_t seg_end = grouped ? seg_begin + seg_len : n_steps;
     // Budgeted rounds escalate after budget_steps (< n_steps); segments are
     // complete after seg_len steps.
     const uint32_t limit = split ? seg_len : args.budget_steps;
@@ -713,7 +733,7 @@ __global__ __launch_bounds__(kBlock) void EbMfmaKernel(MfmaArgs args) {
     // kFilterSteps MFMAs); the next iteration's tiles are requested at the
     // top of each iteration.
     auto next_step = [&](uint32_t st) __attribute__((always_inline)) {
-      return st + 1u == seg_end ? seg_begin : st + 1u;
+      return st + seg_stride >= seg_end ? seg_begin : st + seg_stride;
     };
     float bcur[4 * kFilterSteps], bnext[4 * kFilterSteps];
     uint32_t nxt = step;  // first step of the next iteration

@@ -8,10 +8,13 @@ its scan starts to the first of its "near" B points. This model measures,
 for a sample of A points of the clustered pair:
 
 * mean: the cost of one scan from a random start (the budgeted rounds);
-* seg16: the cost when B is cut into 16 contiguous segments scanned side by
-  side, which stop together at the first hit in any of them (the tail's
-  segmented scans), i.e. 16 x the smallest offset of a near point within
-  its segment;
+* segC / segI: the cost of a segmented scan (the seed, top completions and
+  tail): B's 64-point steps dealt to W segments scanned side by side, which
+  stop together at the first hit in any of them. segC cuts the steps into
+  W contiguous ranges (W x 64 x the smallest step offset of a near point
+  within its range); segI deals them in turn, segment s taking steps s,
+  s + W, ... (W x 64 x the smallest step index of a near point divided by
+  W), as the library does;
 
 for these orders:
 
@@ -26,7 +29,8 @@ for these orders:
   wrong way round).
 
 Run: python oracle/b_order_model.py [--n 1000000] [--sample 20000]
-(needs scipy for the near sets).
+[--segs 16] (needs scipy for the near sets). "hard" averages over the
+sampled points with at most 30 near B points (the tail's kind).
 """
 from __future__ import annotations
 
@@ -103,6 +107,7 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--sample", type=int, default=20_000)
+    ap.add_argument("--segs", type=int, default=16)
     args = ap.parse_args()
     n = args.n
     a, b = gen.gmm_pair(n)
@@ -113,16 +118,25 @@ def main() -> None:
     sample = rng.choice(n, min(n, args.sample), replace=False)
     near = tree.query_ball_point(a[sample].astype(np.float64), np.sqrt(cmax2) * (1 - 1e-9))
     starts = rng.integers(0, n, len(sample))
-    seg = -(-n // 16)
-    print(f"|A| = |B| = {n}, final cmax2 = {cmax2:.6g}, sample of {len(sample)} A points")
+    w = args.segs
+    n_steps = -(-n // 64)
+    seg_steps = -(-n_steps // w)
+    hard = np.array([len(nb) <= 30 for nb in near])
+    print(f"|A| = |B| = {n}, final cmax2 = {cmax2:.6g}, sample of {len(sample)} A points "
+          f"({hard.sum()} hard), {w} segments")
     for name, o in orders(b, rng).items():
         pos = np.empty(n, np.int64)
         pos[o] = np.arange(n)
         mean = np.mean([((pos[np.asarray(nb)] - s) % n).min() + 1 if nb else n
                         for nb, s in zip(near, starts)])
-        seg16 = np.mean([16 * ((pos[np.asarray(nb)] % seg).min() + 1) if nb else n
-                         for nb in near])
-        print(f"{name:10s} mean {mean:9.1f}  seg16 {seg16:9.1f} evaluations per A point")
+        seg_c, seg_i = [], []
+        for nb in near:
+            st = pos[np.asarray(nb, np.int64)] // 64
+            seg_c.append(w * 64 * ((st % seg_steps).min() + 1) if nb else n)
+            seg_i.append(w * 64 * ((st // w).min() + 1) if nb else n)
+        seg_c, seg_i = np.array(seg_c), np.array(seg_i)
+        print(f"{name:10s} mean {mean:9.1f}  segC {seg_c.mean():9.1f} (hard {seg_c[hard].mean():9.1f})"
+              f"  segI {seg_i.mean():9.1f} (hard {seg_i[hard].mean():9.1f}) evaluations per A point")
 
 
 if __name__ == "__main__":
