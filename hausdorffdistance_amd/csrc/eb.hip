@@ -1080,9 +1080,11 @@ int LaunchScan(hd_context* ctx, ScanArgs& args, uint32_t pos_begin,
 
 // Exact minima of the entries of `list` with B split into segments scanned
 // by different waves: as many as keep every slot busy, at least min_segs,
-// at most 256 steps each (long segments leave the launch to a few slow
-// waves: the 2048-point seed of the learned order takes 1.7 ms with 1953-step
-// segments, 1.4 ms with 256), at least 64 steps each.
+// at most 512 steps each, at least 64 steps each. Long segments leave the
+// launch to a few slow waves, short ones add the steps a segment spends
+// after another's hit: the 2048-point seed of the learned order on the 10M
+// GMM takes 1.35 ms with 128-step segments, 1.08 ms with 256, 1.06 ms
+// with 512 (1.7 ms with 1953-step contiguous ranges).
 template <int ND>
 int LaunchSplit(hd_context* ctx, const float* a, const uint32_t* list,
                 const float* list_min, uint32_t n_list, const float* bx,
@@ -1093,7 +1095,7 @@ int LaunchSplit(hd_context* ctx, const float* a, const uint32_t* list,
   const uint32_t total_slots =
       (uint32_t)ResidentBlocks<ND>(ctx) * (kBlock / kWaveSize) * kP;
   const uint32_t want = (4u * total_slots + n_list - 1) / n_list;
-  uint32_t n_segs = std::max(std::max(min_segs, want), n_steps / 256u);
+  uint32_t n_segs = std::max(std::max(min_segs, want), n_steps / 512u);
   n_segs = std::min(n_segs, std::max(1u, n_steps / 64u));
   const uint32_t seg_steps = (n_steps + n_segs - 1) / n_segs;
   n_segs = (n_steps + seg_steps - 1) / seg_steps;
